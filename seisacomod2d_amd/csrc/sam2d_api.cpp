@@ -1,0 +1,701 @@
+// sam2d_api.cpp -- the C ABI of include/sam2d.h: contexts, resident device
+// domains and shot state, the captured time loop, batched tile lists and the
+// autotuner.  The gradient entry point lives in sam2d_adjoint.cpp.
+#include "sam2d_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+using namespace sam2d;
+
+namespace sam2d {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+const char *last_error() { return g_last_error.c_str(); }
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------- device memory
+
+int dev_alloc(void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return SAM2D_OK;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(SAM2D_ERR_MEMORY, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return SAM2D_OK;
+}
+
+void dev_free(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+int upload(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return SAM2D_OK;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return SAM2D_OK;
+}
+
+DeviceDomain::~DeviceDomain()
+{
+    dev_free(KX); if (KZ != KX) dev_free(KZ);
+    dev_free(BX); dev_free(BZ); dev_free(prof_x); dev_free(prof_z);
+}
+
+ShotBuffers::~ShotBuffers() { release(); }
+
+void ShotBuffers::release()
+{
+    for (int b = 0; b < 2; b++)
+        for (int f = 0; f < 4; f++) { dev_free(st[b][f]); st[b][f] = nullptr; }
+    dev_free(amp); dev_free(gcell); dev_free(seis);
+    amp = nullptr; gcell = nullptr; seis = nullptr;
+    cap_cells = 0; cap_nts = 0; cap_ng = 0; cap_seis = 0;
+}
+
+int ShotBuffers::reserve(size_t cells, int nts, int ng, size_t nseis)
+{
+    if (cells > cap_cells) {
+        for (int b = 0; b < 2; b++)
+            for (int f = 0; f < 4; f++) {
+                dev_free(st[b][f]); st[b][f] = nullptr;
+                SAM2D_TRY(dev_alloc((void **)&st[b][f], cells * sizeof(float)));
+            }
+        cap_cells = cells;
+    }
+    if (nts > cap_nts) {
+        dev_free(amp); amp = nullptr;
+        SAM2D_TRY(dev_alloc((void **)&amp, (size_t)nts * sizeof(float)));
+        cap_nts = nts;
+    }
+    if (ng > cap_ng) {
+        dev_free(gcell); gcell = nullptr;
+        SAM2D_TRY(dev_alloc((void **)&gcell, (size_t)ng * sizeof(int)));
+        cap_ng = ng;
+    }
+    if (nseis > cap_seis) {
+        dev_free(seis); seis = nullptr;
+        SAM2D_TRY(dev_alloc((void **)&seis, nseis * sizeof(float)));
+        cap_seis = nseis;
+    }
+    return SAM2D_OK;
+}
+
+// ------------------------------------------------------------------ domains
+
+int get_domain(sam2d_ctx *ctx, int wx0, int w, DeviceDomain **out)
+{
+    const long long key = ((long long)wx0 << 32) | (unsigned)w;
+    auto it = ctx->domains.find(key);
+    if (it != ctx->domains.end()) { *out = it->second.get(); return SAM2D_OK; }
+    std::unique_ptr<DeviceDomain> d(new DeviceDomain());
+    build_domain(ctx->grid, ctx->ta, ctx->vp.data(), ctx->rho.data(), wx0, w, d->host);
+    const DomainHost &h = d->host;
+    const size_t N = (size_t)h.nnx * h.nnz;
+    SAM2D_TRY(dev_alloc((void **)&d->KX, N * 4));
+    SAM2D_TRY(dev_alloc((void **)&d->BX, N * 4));
+    SAM2D_TRY(dev_alloc((void **)&d->BZ, N * 4));
+    if (h.kz_alias) d->KZ = d->KX;
+    else SAM2D_TRY(dev_alloc((void **)&d->KZ, N * 4));
+    SAM2D_TRY(dev_alloc((void **)&d->prof_x, h.prof_x.size() * 4));
+    SAM2D_TRY(dev_alloc((void **)&d->prof_z, h.prof_z.size() * 4));
+    SAM2D_TRY(upload(d->KX, h.KX.data(), N * 4, ctx->stream));
+    if (!h.kz_alias) SAM2D_TRY(upload(d->KZ, h.KZ.data(), N * 4, ctx->stream));
+    SAM2D_TRY(upload(d->BX, h.BX.data(), N * 4, ctx->stream));
+    SAM2D_TRY(upload(d->BZ, h.BZ.data(), N * 4, ctx->stream));
+    SAM2D_TRY(upload(d->prof_x, h.prof_x.data(), h.prof_x.size() * 4, ctx->stream));
+    SAM2D_TRY(upload(d->prof_z, h.prof_z.data(), h.prof_z.size() * 4, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    // The FP32 host copies are no longer needed; kappa stays for the source.
+    d->host.KX = std::vector<float>(); d->host.KZ = std::vector<float>();
+    d->host.BX = std::vector<float>(); d->host.BZ = std::vector<float>();
+    *out = d.get();
+    ctx->domains[key] = std::move(d);
+    return SAM2D_OK;
+}
+
+// Validate a shot against its domain and fill the host part of its plan.
+int plan_shot(sam2d_ctx *ctx, const sam2d_shot &sh, DeviceDomain *dom, ShotPlan &p)
+{
+    const sam2d_grid &g = ctx->grid;
+    const DomainHost &h = dom->host;
+    p.dom = dom;
+    p.ng = sh.nrec;
+    if (sh.nrec < 0) return fail(SAM2D_ERR_PARAM, "nrec < 0");
+    if (sh.nrec > 0 && !sh.rec_xz) return fail(SAM2D_ERR_PARAM, "rec_xz is NULL");
+    int ix, iz;
+    coord_to_cell(g, sh.sx, sh.sz, ix, iz);
+    if (ix < h.wx0 || ix >= h.wx0 + h.w || iz < 0 || iz >= g.nz)
+        return fail(SAM2D_ERR_RECEIVER, "source (%g, %g) outside the modelling domain", sh.sx, sh.sz);
+    p.src_x = ix - h.wx0 + g.npml;
+    p.src_z = iz + g.npml;
+    p.gcell.resize(sh.nrec);
+    for (int k = 0; k < sh.nrec; k++) {
+        coord_to_cell(g, sh.rec_xz[2 * k], sh.rec_xz[2 * k + 1], ix, iz);
+        if (ix < h.wx0 || ix >= h.wx0 + h.w || iz < 0 || iz >= g.nz)
+            return fail(SAM2D_ERR_RECEIVER, "receiver %d (%g, %g) outside the modelling domain", k,
+                        sh.rec_xz[2 * k], sh.rec_xz[2 * k + 1]);
+        p.gcell[k] = (ix - h.wx0 + g.npml) * h.nnz + (iz + g.npml);
+    }
+    // Source increments, cpumodelling.cpp:136-140: fd_dt*w[it]*kappa[src]/(dx*dz)
+    // evaluated in FP64 and rounded once.
+    const double ks = h.kappa[(size_t)p.src_x * h.nnz + p.src_z];
+    const int nts = ctx->ta.nts;
+    p.amp.resize(nts);
+    const double area = (double)(g.dx * g.dz);
+    for (int n = 0; n < nts; n++) p.amp[n] = (float)(g.fd_dt * ctx->wavelet[n] * ks / area);
+    return SAM2D_OK;
+}
+
+// Tile geometry of one shot for a kernel variant.
+void tile_shape(int nnz, int nnx, int waves, int chunk, int &ngroups, int &nchunks)
+{
+    const int nstrips = (nnz + OWN_Z - 1) / OWN_Z;
+    ngroups = (nstrips + waves - 1) / waves;
+    nchunks = (nnx + chunk - 1) / chunk;
+}
+
+int pick_variant(sam2d_ctx *ctx)
+{
+    if (ctx->variant >= 0) return ctx->variant;
+    if (ctx->tuned_variant >= 0) return ctx->tuned_variant;
+    return ctx->default_variant;
+}
+
+// Make `plans` resident: buffers, descriptors, zeroed state and seismograms.
+int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *ntiles_out)
+{
+    int waves, chunk;
+    variant_shape(variant, &waves, &chunk);
+    const int ns = (int)plans.size();
+    if ((int)ctx->bufs.size() < ns) ctx->bufs.resize(ns);
+    std::vector<ShotDev> desc(ns);
+    int tiles = 0;
+    const int real_nt = ctx->ta.real_nt, nts = ctx->ta.nts;
+    for (int s = 0; s < ns; s++) {
+        ShotPlan &p = plans[s];
+        DeviceDomain *dom = p.dom;
+        const int nnz = dom->host.nnz, nnx = dom->host.nnx;
+        const size_t N = (size_t)nnz * nnx;
+        if (!ctx->bufs[s]) ctx->bufs[s].reset(new ShotBuffers());
+        ShotBuffers &b = *ctx->bufs[s];
+        SAM2D_TRY(b.reserve(N, nts, p.ng, (size_t)p.ng * real_nt));
+        SAM2D_TRY(upload(b.amp, p.amp.data(), sizeof(float) * nts, ctx->stream));
+        SAM2D_TRY(upload(b.gcell, p.gcell.data(), sizeof(int) * p.ng, ctx->stream));
+        for (int f = 0; f < 4; f++) HIP_TRY(launch_fill(b.st[0][f], N, 0.0f, ctx->stream));
+        if (p.ng) HIP_TRY(launch_fill(b.seis, (size_t)p.ng * real_nt, 0.0f, ctx->stream));
+        ShotDev &d = desc[s];
+        std::memset(&d, 0, sizeof d);
+        d.nnz = nnz; d.nnx = nnx;
+        tile_shape(nnz, nnx, waves, chunk, d.ngroups, d.nchunks);
+        d.chunk = chunk;
+        d.tile_begin = tiles;
+        tiles += d.ngroups * d.nchunks;
+        d.src_x = p.src_x; d.src_z = p.src_z;
+        d.nts = nts; d.ng = p.ng; d.real_nt = real_nt;
+        d.amp = b.amp;
+        d.KX = dom->KX; d.KZ = dom->KZ; d.BX = dom->BX; d.BZ = dom->BZ;
+        d.prof_x = dom->prof_x; d.prof_z = dom->prof_z;
+        for (int bb = 0; bb < 2; bb++)
+            for (int f = 0; f < 4; f++) d.st[bb][f] = b.st[bb][f];
+        d.gcell = b.gcell; d.seis = b.seis;
+        p.seis_dev = b.seis;
+    }
+    if (ns > ctx->cap_desc) {
+        // Captured graphs hold the old descriptor address: drop them.
+        for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+        ctx->graphs.clear();
+        dev_free(ctx->d_desc); ctx->d_desc = nullptr;
+        SAM2D_TRY(dev_alloc((void **)&ctx->d_desc, sizeof(ShotDev) * ns));
+        ctx->cap_desc = ns;
+    }
+    SAM2D_TRY(upload(ctx->d_desc, desc.data(), sizeof(ShotDev) * ns, ctx->stream));
+    ctx->staged_desc = desc;
+    *ntiles_out = tiles;
+    return SAM2D_OK;
+}
+
+// Receiver sample written by step kernel `n` (the state it reads is the
+// result of step n-1): cpumodelling.cpp:355-360 records after step kt when
+// (kt+1) % dt_factor == 0 into sample (kt+1)/dt_factor - 1.
+static int rec_slot_for_kernel(const TimeAxis &ta, int n)
+{
+    if (n < 1 || n % ta.dt_factor != 0) return -1;
+    const int slot = n / ta.dt_factor - 1;
+    return slot < ta.real_nt ? slot : -1;
+}
+
+// Enqueue steps [n0, n1) of the time loop (the tail record when n1 == fd_nt).
+int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int waves, int n0, int n1, int max_ng)
+{
+    for (int n = n0; n < n1; n++)
+        HIP_TRY(launch_step(waves, ctx->d_desc, nshots, ntiles, ctx->grid_cap, n, n & 1,
+                            rec_slot_for_kernel(ctx->ta, n), ctx->stream));
+    if (n1 == ctx->ta.fd_nt) {
+        const int slot = rec_slot_for_kernel(ctx->ta, n1);
+        if (slot >= 0 && max_ng > 0)
+            HIP_TRY(launch_record(ctx->d_desc, nshots, max_ng, n1 & 1, slot, ctx->stream));
+    }
+    return SAM2D_OK;
+}
+
+// Run the whole time loop of the staged batch, as a replayed hipGraph when
+// enabled.  Adds the step-kernel time to ctx->stats.
+int run_time_loop(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int max_ng)
+{
+    int waves, chunk;
+    variant_shape(variant, &waves, &chunk);
+    const int fd_nt = ctx->ta.fd_nt;
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    bool used_graph = false;
+    if (ctx->use_graph) {
+        GraphKey key{nshots, ntiles, variant, max_ng, ctx->grid_cap};
+        hipGraphExec_t exec = nullptr;
+        auto it = ctx->graphs.find(key);
+        if (it != ctx->graphs.end()) exec = it->second;
+        if (!exec) {
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_steps(ctx, nshots, ntiles, waves, 0, fd_nt, max_ng);
+            hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+            if (rc != SAM2D_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+            HIP_TRY(e);
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIP_TRY(e);
+            // The first capture of a shape pays instantiation; the timing below
+            // brackets only the replay.
+            HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+            if (ctx->graphs.size() > 16) {
+                for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+                ctx->graphs.clear();
+            }
+            ctx->graphs[key] = exec;
+        }
+        HIP_TRY(hipGraphLaunch(exec, ctx->stream));
+        used_graph = true;
+    } else {
+        SAM2D_TRY(enqueue_steps(ctx, nshots, ntiles, waves, 0, fd_nt, max_ng));
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.kernel_ms += ms;
+    ctx->stats.launches += fd_nt;
+    ctx->stats.used_graph = used_graph ? 1 : 0;
+    ctx->stats.variant = variant;
+    return SAM2D_OK;
+}
+
+// Forward over a list of shot plans, `batch` shots resident at a time.
+// seis: host (dev_out == false) or device destination, concatenated.
+int forward_plans(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int batch, float *seis, bool dev_out)
+{
+    const int real_nt = ctx->ta.real_nt;
+    size_t off = 0;
+    const int ns = (int)plans.size();
+    for (int s0 = 0; s0 < ns; s0 += batch) {
+        const int s1 = std::min(ns, s0 + batch);
+        std::vector<ShotPlan> sub(plans.begin() + s0, plans.begin() + s1);
+        const int variant = pick_variant(ctx);
+        int ntiles = 0;
+        SAM2D_TRY(stage_batch(ctx, sub, variant, &ntiles));
+        int max_ng = 0;
+        for (auto &p : sub) {
+            max_ng = std::max(max_ng, p.ng);
+            ctx->stats.cell_updates += (double)p.dom->host.nnx * p.dom->host.nnz * ctx->ta.fd_nt;
+        }
+        SAM2D_TRY(run_time_loop(ctx, (int)sub.size(), ntiles, variant, max_ng));
+        for (auto &p : sub) {
+            const size_t n = (size_t)p.ng * real_nt;
+            if (n) HIP_TRY(hipMemcpyAsync(seis + off, p.seis_dev, n * sizeof(float),
+                                          dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                          ctx->stream));
+            off += n;
+        }
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return SAM2D_OK;
+}
+
+// The reference's carry-over between shots (cpumodelling.cpp:343-348): the
+// next shot starts from the previous final state with only the first nnz
+// cells of p, px, pz, vx, vz cleared.  Per-shot loop only, no graph.
+static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, float *seis)
+{
+    const int real_nt = ctx->ta.real_nt;
+    size_t off = 0;
+    const int variant = pick_variant(ctx);
+    int waves, chunk;
+    variant_shape(variant, &waves, &chunk);
+    for (size_t s = 0; s < plans.size(); s++) {
+        std::vector<ShotPlan> one(1, plans[s]);
+        // Keep the previous state: stage into slot 0 without its clearing.
+        ShotBuffers *b0 = ctx->bufs.empty() || !ctx->bufs[0] ? nullptr : ctx->bufs[0].get();
+        const size_t N = (size_t)one[0].dom->host.nnx * one[0].dom->host.nnz;
+        std::vector<float> keep;
+        const int fin = ctx->ta.fd_nt & 1;
+        if (s > 0 && b0) {
+            // Bring the final state into set 0, then clear the first nnz cells.
+            for (int f = 0; f < 4; f++)
+                if (fin) HIP_TRY(hipMemcpyAsync(b0->st[0][f], b0->st[1][f], N * 4, hipMemcpyDeviceToDevice, ctx->stream));
+            keep.resize(4 * N);
+            for (int f = 0; f < 4; f++)
+                HIP_TRY(hipMemcpyAsync(&keep[f * N], b0->st[0][f], N * 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            const int nnz = one[0].dom->host.nnz;
+            for (int f = 0; f < 4; f++) std::fill(&keep[f * N], &keep[f * N] + nnz, 0.0f);
+        }
+        int ntiles = 0;
+        SAM2D_TRY(stage_batch(ctx, one, variant, &ntiles));
+        if (s > 0 && !keep.empty()) {
+            ShotBuffers &b = *ctx->bufs[0];
+            for (int f = 0; f < 4; f++)
+                SAM2D_TRY(upload(b.st[0][f], &keep[f * N], N * 4, ctx->stream));
+        }
+        ctx->stats.cell_updates += (double)N * ctx->ta.fd_nt;
+        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+        SAM2D_TRY(enqueue_steps(ctx, 1, ntiles, waves, 0, ctx->ta.fd_nt, one[0].ng));
+        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(ctx->ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->stats.kernel_ms += ms;
+        ctx->stats.launches += ctx->ta.fd_nt;
+        const size_t n = (size_t)one[0].ng * real_nt;
+        if (n) HIP_TRY(hipMemcpy(seis + off, one[0].seis_dev, n * 4, hipMemcpyDeviceToHost));
+        off += n;
+    }
+    return SAM2D_OK;
+}
+
+int make_plans(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots, const int *windows,
+               std::vector<ShotPlan> &plans)
+{
+    if (nshots < 0) return fail(SAM2D_ERR_PARAM, "nshots < 0");
+    if (nshots > 0 && !shots) return fail(SAM2D_ERR_PARAM, "shots is NULL");
+    plans.assign(nshots, ShotPlan());
+    for (int s = 0; s < nshots; s++) {
+        int wx0 = 0, w = ctx->grid.nx;
+        if (windows) {
+            wx0 = windows[2 * s];
+            w = windows[2 * s + 1];
+            if (wx0 < 0 || w < 1 || wx0 + w > ctx->grid.nx)
+                return fail(SAM2D_ERR_PARAM, "window %d (%d, %d) outside the model (nx=%d)", s, wx0,
+                            w, ctx->grid.nx);
+            if (w + 2 * ctx->grid.npml < 4)
+                return fail(SAM2D_ERR_PARAM, "window %d too narrow", s);
+        }
+        DeviceDomain *dom = nullptr;
+        SAM2D_TRY(get_domain(ctx, wx0, w, &dom));
+        SAM2D_TRY(plan_shot(ctx, shots[s], dom, plans[s]));
+    }
+    return SAM2D_OK;
+}
+
+static size_t seis_total(const sam2d_shot *shots, int nshots, int real_nt)
+{
+    size_t n = 0;
+    for (int s = 0; s < nshots; s++) n += (size_t)shots[s].nrec * real_nt;
+    return n;
+}
+
+}  // namespace sam2d
+
+// ======================================================================= ABI
+
+extern "C" {
+
+const char *sam2d_version(void) { return "sam2d-mi355x 0.1 (gfx950)"; }
+
+const char *sam2d_last_error(void) { return sam2d::last_error(); }
+
+int sam2d_num_variants(void) { return num_variants(); }
+
+const char *sam2d_variant_name(int v)
+{
+    static thread_local char buf[64];
+    if (v < 0 || v >= num_variants()) return "invalid";
+    int w, c;
+    variant_shape(v, &w, &c);
+    snprintf(buf, sizeof buf, "waves%d_chunk%d", w, c);
+    return buf;
+}
+
+int sam2d_check_stability(const sam2d_grid *g, float vmin, float vmax)
+{
+    clear_error();
+    if (!g) return fail(SAM2D_ERR_PARAM, "grid is NULL");
+    // check_stability.cpp:19-64, float arithmetic as there.
+    const float fm = 2.0f * g->dom_freq;
+    const float hmin = g->dx < g->dz ? g->dx : g->dz;
+    const float hmax = g->dx > g->dz ? g->dx : g->dz;
+    float criteria = (0.606f * hmin) / vmax;
+    if (g->fd_dt > criteria)
+        return fail(SAM2D_ERR_STABILITY, "stability condition violated: dt %g > %g", g->fd_dt, criteria);
+    criteria = vmin / (hmax * 5.0f);
+    if (fm > criteria)
+        return fail(SAM2D_ERR_STABILITY, "dispersion condition violated: peak frequency %g > %g",
+                    g->dom_freq, criteria / 2.0f);
+    return SAM2D_OK;
+}
+
+int sam2d_create(const sam2d_grid *g, const float *vp, const float *rho, sam2d_ctx **out)
+{
+    clear_error();
+    if (!out) return fail(SAM2D_ERR_PARAM, "out is NULL");
+    *out = nullptr;
+    if (!g || !vp || !rho) return fail(SAM2D_ERR_PARAM, "NULL grid or model");
+    if (g->nx < 1 || g->nz < 1) return fail(SAM2D_ERR_PARAM, "bad grid size %d x %d", g->nx, g->nz);
+    if (!(g->dx > 0.0f) || !(g->dz > 0.0f)) return fail(SAM2D_ERR_PARAM, "grid spacing must be > 0");
+    if (!(g->fd_dt > 0.0f) || !(g->real_dt > 0.0f) || !(g->rec_len > 0.0f))
+        return fail(SAM2D_ERR_PARAM, "time parameters must be > 0");
+    if (g->npml < 1) return fail(SAM2D_ERR_PARAM, "npml must be >= 1");
+    if (g->fdop != 4)
+        return fail(SAM2D_ERR_PARAM, "only the 4th-order operator is implemented (fdop=%d)", g->fdop);
+    // json_parser.cpp:189-190
+    if (std::fmod(g->real_dt, g->fd_dt) != 0.0f)
+        return fail(SAM2D_ERR_PARAM, "DATA DT must be a multiple of FD DT");
+    if (g->nz + 2 * g->npml < 4 || g->nx + 2 * g->npml < 4)
+        return fail(SAM2D_ERR_PARAM, "padded grid too small");
+    const size_t n = (size_t)g->nx * g->nz;
+    if ((size_t)(g->nz + 2 * g->npml) * (size_t)(g->nx + 2 * g->npml) > (size_t)0x7fffffff)
+        return fail(SAM2D_ERR_PARAM, "padded grid exceeds 2^31 cells");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        return fail(SAM2D_ERR_DEVICE, "no HIP device available");
+    }
+    std::unique_ptr<sam2d_ctx> ctx(new sam2d_ctx());
+    ctx->grid = *g;
+    ctx->ta = time_axis(*g);
+    if (ctx->ta.dt_factor < 1) return fail(SAM2D_ERR_PARAM, "DATA DT < FD DT");
+    ctx->vp.assign(vp, vp + n);
+    ctx->rho.assign(rho, rho + n);
+    ctx->wavelet = ricker(g->dom_freq, g->fd_dt, ctx->ta.nts);
+    HIP_TRY(hipGetDevice(&ctx->device));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&ctx->ev0));
+    HIP_TRY(hipEventCreate(&ctx->ev1));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
+    ctx->num_cu = prop.multiProcessorCount;
+    ctx->default_variant = default_variant_index();
+    DeviceDomain *dom = nullptr;
+    SAM2D_TRY(get_domain(ctx.get(), 0, g->nx, &dom));
+    *out = ctx.release();
+    return SAM2D_OK;
+}
+
+int sam2d_get_dims(const sam2d_ctx *ctx, sam2d_dims *out)
+{
+    if (!ctx || !out) return fail(SAM2D_ERR_PARAM, "NULL argument");
+    out->fd_nt = ctx->ta.fd_nt;
+    out->real_nt = ctx->ta.real_nt;
+    out->dt_factor = ctx->ta.dt_factor;
+    out->nts = ctx->ta.nts;
+    out->nnx = ctx->grid.nx + 2 * ctx->grid.npml;
+    out->nnz = ctx->grid.nz + 2 * ctx->grid.npml;
+    return SAM2D_OK;
+}
+
+static int forward_common(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots, const int *windows,
+                          float *seis, bool dev_out, bool batched)
+{
+    clear_error();
+    if (!ctx) return fail(SAM2D_ERR_PARAM, "ctx is NULL");
+    if (nshots > 0 && !seis && seis_total(shots, nshots, ctx->ta.real_nt) > 0)
+        return fail(SAM2D_ERR_PARAM, "seis is NULL");
+    const double t0 = now_ms();
+    ctx->stats = sam2d_stats();
+    std::vector<ShotPlan> plans;
+    SAM2D_TRY(make_plans(ctx, shots, nshots, windows, plans));
+    if (ctx->reset_quirk && !batched && !dev_out && !windows) {
+        SAM2D_TRY(forward_reset_quirk(ctx, plans, seis));
+    } else {
+        int batch = 1;
+        if (batched) {
+            batch = ctx->max_batch > 0 ? ctx->max_batch : nshots;
+            if (batch < 1) batch = 1;
+            if (ctx->variant < 0 && ctx->tuned_variant < 0 && nshots > 0)
+                SAM2D_TRY(sam2d_autotune(ctx, windows, nshots, 8, nullptr));
+        }
+        SAM2D_TRY(forward_plans(ctx, plans, batch, seis, dev_out));
+    }
+    ctx->stats.wall_ms = now_ms() - t0;
+    return SAM2D_OK;
+}
+
+int sam2d_forward(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots, float *seis)
+{
+    return forward_common(ctx, shots, nshots, nullptr, seis, false, false);
+}
+
+int sam2d_forward_dev(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots, float *seis_dev)
+{
+    return forward_common(ctx, shots, nshots, nullptr, seis_dev, true, false);
+}
+
+int sam2d_forward_windows(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots, const int *windows,
+                          float *seis)
+{
+    return forward_common(ctx, shots, nshots, windows, seis, false, false);
+}
+
+int sam2d_forward_batched(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots, const int *windows,
+                          float *seis)
+{
+    return forward_common(ctx, shots, nshots, windows, seis, false, true);
+}
+
+int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value)
+{
+    clear_error();
+    if (!ctx || !key) return fail(SAM2D_ERR_PARAM, "NULL argument");
+    std::string k(key);
+    if (k == "graph") ctx->use_graph = value != 0;
+    else if (k == "variant") {
+        if (value >= num_variants()) return fail(SAM2D_ERR_PARAM, "variant %d out of range", value);
+        ctx->variant = value < 0 ? -1 : value;
+    } else if (k == "reset_quirk") ctx->reset_quirk = value != 0;
+    else if (k == "max_batch") ctx->max_batch = value < 0 ? 0 : value;
+    else if (k == "grid_cap") ctx->grid_cap = value < 0 ? 0 : value;
+    else return fail(SAM2D_ERR_PARAM, "unknown option '%s'", key);
+    return SAM2D_OK;
+}
+
+int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, int *best_variant)
+{
+    clear_error();
+    if (!ctx) return fail(SAM2D_ERR_PARAM, "ctx is NULL");
+    if (steps < 1) steps = 4;
+    if (nshots < 1) nshots = 1;
+    // Synthetic shots at the centre of each window (or of the model), no
+    // receivers: the tuner times the step kernel alone.
+    std::vector<sam2d_shot> shots(nshots);
+    for (int s = 0; s < nshots; s++) {
+        int wx0 = windows ? windows[2 * s] : 0;
+        int w = windows ? windows[2 * s + 1] : ctx->grid.nx;
+        shots[s].sx = (float)(wx0 + w / 2) * ctx->grid.dx;
+        shots[s].sz = 0.0f;
+        shots[s].nrec = 0;
+        shots[s].rec_xz = nullptr;
+    }
+    std::vector<ShotPlan> plans;
+    SAM2D_TRY(make_plans(ctx, shots.data(), nshots, windows, plans));
+    int batch = ctx->max_batch > 0 ? std::min(ctx->max_batch, nshots) : nshots;
+    plans.resize(batch);
+    double best_ms = 1e300;
+    int best = default_variant_index();
+    for (int v = 0; v < num_variants(); v++) {
+        int waves, chunk;
+        variant_shape(v, &waves, &chunk);
+        int ntiles = 0;
+        SAM2D_TRY(stage_batch(ctx, plans, v, &ntiles));
+        SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, waves, 0, 2, 0));   // warm
+        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+        SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, waves, 2, 2 + steps, 0));
+        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(ctx->ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->tune_ms[v] = ms / steps;
+        if (ms < best_ms) { best_ms = ms; best = v; }
+    }
+    ctx->tuned_variant = best;
+    if (best_variant) *best_variant = best;
+    return SAM2D_OK;
+}
+
+int sam2d_coefficients(const sam2d_grid *g, const float *vp, const float *rho, int wx0, int w,
+                       float *KX, float *KZ, float *BX, float *BZ, float *prof_x, float *prof_z)
+{
+    clear_error();
+    if (!g || !vp || !rho) return fail(SAM2D_ERR_PARAM, "NULL argument");
+    if (wx0 < 0 || w < 1 || wx0 + w > g->nx) return fail(SAM2D_ERR_PARAM, "bad window");
+    const TimeAxis ta = time_axis(*g);
+    DomainHost h;
+    build_domain(*g, ta, vp, rho, wx0, w, h);
+    const size_t N = (size_t)h.nnx * h.nnz;
+    if (KX) std::memcpy(KX, h.KX.data(), N * 4);
+    if (KZ) std::memcpy(KZ, h.kz_alias ? h.KX.data() : h.KZ.data(), N * 4);
+    if (BX) std::memcpy(BX, h.BX.data(), N * 4);
+    if (BZ) std::memcpy(BZ, h.BZ.data(), N * 4);
+    if (prof_x) std::memcpy(prof_x, h.prof_x.data(), h.prof_x.size() * 4);
+    if (prof_z) std::memcpy(prof_z, h.prof_z.data(), h.prof_z.size() * 4);
+    return SAM2D_OK;
+}
+
+int sam2d_source_increments(const sam2d_grid *g, const float *vp, const float *rho, float sx,
+                            float sz, float *amp)
+{
+    clear_error();
+    if (!g || !vp || !rho || !amp) return fail(SAM2D_ERR_PARAM, "NULL argument");
+    sam2d_ctx tmp;
+    tmp.grid = *g;
+    tmp.ta = time_axis(*g);
+    tmp.wavelet = ricker(g->dom_freq, g->fd_dt, tmp.ta.nts);
+    tmp.vp.assign(vp, vp + (size_t)g->nx * g->nz);
+    tmp.rho.assign(rho, rho + (size_t)g->nx * g->nz);
+    DeviceDomain dom;
+    build_domain(*g, tmp.ta, vp, rho, 0, g->nx, dom.host);
+    sam2d_shot sh{sx, sz, 0, nullptr};
+    ShotPlan p;
+    SAM2D_TRY(plan_shot(&tmp, sh, &dom, p));
+    std::memcpy(amp, p.amp.data(), p.amp.size() * 4);
+    return SAM2D_OK;
+}
+
+int sam2d_get_stats(const sam2d_ctx *ctx, sam2d_stats *out)
+{
+    if (!ctx || !out) return fail(SAM2D_ERR_PARAM, "NULL argument");
+    *out = ctx->stats;
+    return SAM2D_OK;
+}
+
+void sam2d_destroy(sam2d_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+    ctx->bufs.clear();
+    ctx->domains.clear();
+    dev_free(ctx->d_desc);
+    sam2d_adjoint_release(ctx);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+}  // extern "C"

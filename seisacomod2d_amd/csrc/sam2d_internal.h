@@ -1,0 +1,123 @@
+// sam2d_internal.h -- context and helpers shared by the library's host files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "../../include/sam2d.h"
+#include "sam2d_device.h"
+#include "sam2d_setup.h"
+
+namespace sam2d {
+
+int fail(int code, const char *fmt, ...);
+void clear_error();
+const char *last_error();
+double now_ms();
+
+#define SAM2D_TRY(expr)                  \
+    do {                                 \
+        int _rc = (expr);                \
+        if (_rc != SAM2D_OK) return _rc; \
+    } while (0)
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess) {                                                           \
+            (void)hipGetLastError();                                                      \
+            return ::sam2d::fail(SAM2D_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr,        \
+                                 hipGetErrorString(_e), __FILE__, __LINE__);              \
+        }                                                                                 \
+    } while (0)
+
+int dev_alloc(void **p, size_t bytes);
+void dev_free(void *p);
+int upload(void *dst, const void *src, size_t bytes, hipStream_t s);
+
+// Coefficients of one padded domain on the device.
+struct DeviceDomain {
+    DomainHost host;
+    float *KX = nullptr, *KZ = nullptr, *BX = nullptr, *BZ = nullptr;
+    float *prof_x = nullptr, *prof_z = nullptr;
+    ~DeviceDomain();
+};
+
+// Device state of one resident shot slot (grown on demand, reused).
+struct ShotBuffers {
+    float *st[2][4] = {{nullptr}};
+    float *amp = nullptr;
+    int *gcell = nullptr;
+    float *seis = nullptr;
+    size_t cap_cells = 0, cap_seis = 0;
+    int cap_nts = 0, cap_ng = 0;
+    int reserve(size_t cells, int nts, int ng, size_t nseis);
+    void release();
+    ~ShotBuffers();
+};
+
+// Host-side plan of one shot.
+struct ShotPlan {
+    DeviceDomain *dom = nullptr;
+    int src_x = 0, src_z = 0;      // padded indices in its domain
+    int ng = 0;
+    std::vector<int> gcell;
+    std::vector<float> amp;
+    float *seis_dev = nullptr;
+};
+
+struct GraphKey {
+    int nshots, ntiles, variant, max_ng, grid_cap;
+    bool operator<(const GraphKey &o) const
+    {
+        return std::tie(nshots, ntiles, variant, max_ng, grid_cap) <
+               std::tie(o.nshots, o.ntiles, o.variant, o.max_ng, o.grid_cap);
+    }
+};
+
+struct AdjointState;
+
+int default_variant_index();
+
+}  // namespace sam2d
+
+struct sam2d_ctx {
+    sam2d_grid grid;
+    sam2d::TimeAxis ta;
+    std::vector<float> vp, rho;            // host model, x-major
+    std::vector<double> wavelet;           // FP64 Ricker, nts samples
+    int device = 0, num_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::map<long long, std::unique_ptr<sam2d::DeviceDomain>> domains;
+    std::vector<std::unique_ptr<sam2d::ShotBuffers>> bufs;
+    sam2d::ShotDev *d_desc = nullptr;
+    int cap_desc = 0;
+    std::vector<sam2d::ShotDev> staged_desc;
+    std::map<sam2d::GraphKey, hipGraphExec_t> graphs;
+    bool use_graph = true;
+    bool reset_quirk = false;
+    int variant = -1, tuned_variant = -1, default_variant = 0;
+    int max_batch = 0;
+    int grid_cap = 0;
+    double tune_ms[64] = {0};
+    sam2d_stats stats = sam2d_stats();
+    sam2d::AdjointState *adj = nullptr;
+};
+
+namespace sam2d {
+int get_domain(sam2d_ctx *ctx, int wx0, int w, DeviceDomain **out);
+int plan_shot(sam2d_ctx *ctx, const sam2d_shot &sh, DeviceDomain *dom, ShotPlan &p);
+int make_plans(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots, const int *windows,
+               std::vector<ShotPlan> &plans);
+int pick_variant(sam2d_ctx *ctx);
+int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *ntiles_out);
+int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int waves, int n0, int n1, int max_ng);
+void tile_shape(int nnz, int nnx, int waves, int chunk, int &ngroups, int &nchunks);
+}  // namespace sam2d
+
+void sam2d_adjoint_release(sam2d_ctx *ctx);

@@ -1,0 +1,71 @@
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
+
+
+def golden_cases():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load_golden(name):
+    """Inputs and reference output of one golden case (see tests/golden/make_goldens.py)."""
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    nx, nz, dx, dz, T, fd_dt, real_dt, fm = z["params"]
+    nx, nz = int(nx), int(nz)
+    if "vp" in z:
+        vp, rho = z["vp"], z["rho"]
+    else:  # analytic two-layer model of config 0
+        vp = np.full((nx, nz), 1500, np.float32); vp[:, nz // 2:] = 3000
+        rho = np.full((nx, nz), 1000, np.float32); rho[:, nz // 2:] = 2000
+    return dict(nx=nx, nz=nz, dx=float(dx), dz=float(dz), rec_len=float(T), fd_dt=float(fd_dt),
+                real_dt=float(real_dt), fm=float(fm), vp=vp, rho=rho, src_xz=z["src_xz"],
+                rec_xz=z["rec_xz"], seis=z["seis"], trace_idx=z["trace_idx"])
+
+
+# Seismogram comparison used by every parity test.
+#   per-trace relative L2 error <= tol for traces whose L2 norm is at least
+#   FLOOR x the largest trace norm of the gather; quieter traces (numerical
+#   precursors far from the source, ill-conditioned as a relative measure)
+#   are held to the same tol relative to the largest trace norm instead.
+FLOOR = 1e-3
+
+
+def trace_errors(test, ref):
+    test = np.asarray(test, np.float64)
+    ref = np.asarray(ref, np.float64)
+    err = np.linalg.norm(test - ref, axis=-1)
+    nrm = np.linalg.norm(ref, axis=-1)
+    mx = nrm.max() if nrm.size else 0.0
+    scale = np.maximum(nrm, FLOOR * mx)
+    scale = np.where(scale > 0, scale, 1.0)
+    return err / scale
+
+
+def assert_seis_close(test, ref, tol, what=""):
+    assert test.shape == ref.shape, (test.shape, ref.shape)
+    assert np.all(np.isfinite(test)), f"{what}: non-finite samples"
+    e = trace_errors(test, ref)
+    worst = int(np.argmax(e)) if e.size else 0
+    assert e.size == 0 or e.max() <= tol, (
+        f"{what}: worst trace {worst} relative L2 error {e.max():.3e} > {tol:g}")
+    return e
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import pyoracle
+    pyoracle.lib()
+    return pyoracle

@@ -1,0 +1,188 @@
+"""GPU parity of the HIP forward path (through the C ABI) against the
+reference's golden seismograms and the FP64 oracle.
+
+Tolerance (stated by the hot-path specification): relative L2 error per trace
+<= 1e-5 for FP32 state against FP64 results; traces quieter than 1e-3 of the
+gather's largest trace norm are held to 1e-5 of that norm (conftest.FLOOR).
+Paths that must do identical FP32 arithmetic (kernel variants, graph or not,
+batched or per-shot) are compared bit for bit.
+"""
+import numpy as np
+import pytest
+
+import seisacomod2d_amd as sam
+from conftest import assert_seis_close, golden_cases, load_golden
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _grid(g, npml=32, **kw):
+    d = dict(nx=g["nx"], nz=g["nz"], dx=g["dx"], dz=g["dz"], rec_len=g["rec_len"], fd_dt=g["fd_dt"],
+             real_dt=g["real_dt"], dom_freq=g["fm"], npml=npml)
+    d.update(kw)
+    return sam.Grid(**d)
+
+
+def _shot(g):
+    return sam.Shot(float(g["src_xz"][0, 0]), float(g["src_xz"][0, 1]), g["rec_xz"])
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_forward_matches_reference_goldens(case):
+    g = load_golden(case)
+    print(f"[gpu] golden {case} nx={g['nx']} nz={g['nz']}", flush=True)
+    with sam.Modeller(_grid(g), g["vp"], g["rho"]) as m:
+        seis = m.forward([_shot(g)])[0][g["trace_idx"]]
+    e = assert_seis_close(seis, g["seis"], TOL, case)
+    print(f"[gpu] {case}: max trace error {e.max():.2e}", flush=True)
+
+
+def test_all_kernel_variants_bitwise_identical():
+    g = load_golden("hetero_explicit")
+    ref = None
+    with sam.Modeller(_grid(g), g["vp"], g["rho"]) as m:
+        for v in range(len(sam.variants())):
+            m.set_option("variant", v)
+            s = m.forward([_shot(g)])[0]
+            if ref is None:
+                ref = s
+            assert np.array_equal(s.view(np.int32), ref.view(np.int32)), sam.variants()[v]
+
+
+def test_graph_and_plain_launches_bitwise_identical():
+    g = load_golden("salt_small")
+    with sam.Modeller(_grid(g), g["vp"], g["rho"]) as m:
+        a = m.forward([_shot(g)])[0]
+        assert m.stats()["used_graph"]
+        # replay the captured graph a second time: same bits
+        a2 = m.forward([_shot(g)])[0]
+        m.set_option("graph", 0)
+        b = m.forward([_shot(g)])[0]
+        assert not m.stats()["used_graph"]
+    assert np.array_equal(a, a2) and np.array_equal(a, b)
+
+
+def test_multishot_per_shot_loop_vs_oracle(oracle):
+    g = load_golden("hetero_explicit")
+    shots = [sam.Shot(300.0, 100.0, g["rec_xz"]), sam.Shot(1200.0, 50.0, g["rec_xz"][::2]),
+             sam.Shot(0.0, 0.0, g["rec_xz"][:7])]
+    with sam.Modeller(_grid(g, npml=20), g["vp"], g["rho"]) as m:
+        out = m.forward(shots)
+    P = oracle.params(g["nx"], g["nz"], g["dx"], g["dz"], g["rec_len"], g["fd_dt"], g["real_dt"],
+                      g["fm"], 20)
+    ref = oracle.forward(P, g["vp"], g["rho"], [[s.sx, s.sz] for s in shots],
+                         [s.rec_xz for s in shots])
+    for i, (a, b) in enumerate(zip(out, ref)):
+        assert_seis_close(a, b, TOL, f"shot {i}")
+
+
+def test_reset_quirk_vs_oracle(oracle):
+    g = load_golden("tiny_twolayer")
+    shots = [sam.Shot(300.0, 20.0, g["rec_xz"]), sam.Shot(600.0, 20.0, g["rec_xz"])]
+    with sam.Modeller(_grid(g), g["vp"], g["rho"]) as m:
+        m.set_option("reset_quirk", 1)
+        out = m.forward(shots)
+    P = oracle.params(g["nx"], g["nz"], g["dx"], g["dz"], g["rec_len"], g["fd_dt"], g["real_dt"],
+                      g["fm"], 32, reset_quirk=1)
+    ref = oracle.forward(P, g["vp"], g["rho"], [[s.sx, s.sz] for s in shots],
+                         [s.rec_xz for s in shots])
+    for a, b in zip(out, ref):
+        assert_seis_close(a, b, TOL, "quirk")
+
+
+def _windows_case():
+    g = load_golden("salt_small")
+    rng = np.random.default_rng(5)
+    shots, wins = [], []
+    for k in range(5):
+        w = int(rng.integers(60, 150))
+        wx0 = int(rng.integers(0, g["nx"] - w))
+        xs = (wx0 + w // 2) * g["dx"]
+        rec = np.stack([(wx0 + np.arange(w)) * g["dx"], np.full(w, 7.62)], 1)
+        shots.append(sam.Shot(xs, 7.62 * 3, rec))
+        wins.append((wx0, w))
+    return g, shots, wins
+
+
+def test_batched_equals_per_shot_windows_bitwise():
+    g, shots, wins = _windows_case()
+    with sam.Modeller(_grid(g, npml=24), g["vp"], g["rho"]) as m:
+        plain = m.forward_windows(shots, wins)
+        batched = m.forward_batched(shots, wins)
+        st = m.stats()
+        assert st["used_graph"]
+        m.set_option("max_batch", 2)
+        sub = m.forward_batched(shots, wins)
+    for a, b, c in zip(plain, batched, sub):
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
+        assert np.array_equal(a.view(np.int32), c.view(np.int32))
+
+
+def test_windowed_shots_vs_oracle_on_cropped_model(oracle):
+    g, shots, wins = _windows_case()
+    with sam.Modeller(_grid(g, npml=24), g["vp"], g["rho"]) as m:
+        out = m.forward_batched(shots, wins)
+    f32 = np.float32
+    inv_dx = f32(1) / f32(g["dx"])
+    for s, (wx0, w), a in zip(shots, wins, out):
+        P = oracle.params(w, g["nz"], g["dx"], g["dz"], g["rec_len"], g["fd_dt"], g["real_dt"],
+                          g["fm"], 24)
+        # Same cells in the cropped model: column (int)(x * (1/dx)) - wx0,
+        # passed as the cell centre so the oracle's truncation finds it.
+        rec = s.rec_xz.copy()
+        ix = (rec[:, 0].astype(f32) * inv_dx).astype(np.int64)
+        rec[:, 0] = ((ix - wx0) + 0.5) * g["dx"]
+        sx = (int(f32(s.sx) * inv_dx) - wx0 + 0.5) * g["dx"]
+        ref = oracle.forward(P, g["vp"][wx0:wx0 + w], g["rho"][wx0:wx0 + w], [[sx, s.sz]], [rec])[0]
+        assert_seis_close(a, ref, TOL, f"window {wx0},{w}")
+
+
+def test_config0_npml40_vs_oracle(oracle):
+    # Config 0 as specified (sponge 40 cells, which the reference cannot be
+    # built with): 1000 x 1000, 2 s, 1000 receivers every cell at z = 20 m.
+    nx = nz = 1000
+    vp = np.full((nx, nz), 1500, np.float32); vp[:, 500:] = 3000
+    rho = np.full((nx, nz), 1000, np.float32); rho[:, 500:] = 2000
+    rec = np.stack([np.arange(nx) * 10.0, np.full(nx, 20.0)], 1)
+    grid = sam.Grid(nx, nz, 10.0, 10.0, 2.0, 0.001, 0.001, 15.0, npml=40)
+    print("[gpu] config0 npml40", flush=True)
+    with sam.Modeller(grid, vp, rho) as m:
+        out = m.forward([sam.Shot(5000.0, 20.0, rec)])[0]
+    P = oracle.params(nx, nz, 10, 10, 2.0, 0.001, 0.001, 15, 40)
+    ref = oracle.forward(P, vp, rho, [[5000.0, 20.0]], [rec])[0]
+    e = assert_seis_close(out, ref, TOL, "config0 npml40")
+    print(f"[gpu] config0 npml40 max trace error {e.max():.2e}", flush=True)
+
+
+def test_config1_grid_short_record_vs_oracle(oracle):
+    # Config 1's full 5000 x 5000 grid (sponge 50, 26 M cells) over the first
+    # 60 steps: the whole padded domain is exercised at its real size; the
+    # complete 4 s record is covered by the benchmark, not here.
+    nx = nz = 5000
+    vp = np.full((nx, nz), 1500, np.float32); vp[:, 250:] = 3000
+    rho = np.full((nx, nz), 1000, np.float32); rho[:, 250:] = 2000
+    rec = np.stack([np.arange(0, nx, 7) * 10.0, np.full(len(range(0, nx, 7)), 20.0)], 1)
+    grid = sam.Grid(nx, nz, 10.0, 10.0, 0.06, 0.001, 0.001, 15.0, npml=50)
+    print("[gpu] config1 grid, 60 steps", flush=True)
+    with sam.Modeller(grid, vp, rho) as m:
+        out = m.forward([sam.Shot(49990.0 / 2, 20.0, rec)])[0]
+    P = oracle.params(nx, nz, 10, 10, 0.06, 0.001, 0.001, 15, 50)
+    ref = oracle.forward(P, vp, rho, [[49990.0 / 2, 20.0]], [rec])[0]
+    assert_seis_close(out, ref, TOL, "config1 grid")
+
+
+def test_errors_are_statuses():
+    g = load_golden("tiny_twolayer")
+    with sam.Modeller(_grid(g), g["vp"], g["rho"]) as m:
+        with pytest.raises(sam.SAM2DError) as e:
+            m.forward([sam.Shot(5000.0, 20.0, g["rec_xz"])])   # source outside
+        assert e.value.code == -3
+        with pytest.raises(sam.SAM2DError) as e:
+            m.forward_batched([_shot(g)], [(50, 100)])          # window past nx
+        assert e.value.code == -6
+        # the context still works after errors
+        s = m.forward([_shot(g)])[0]
+    assert_seis_close(s, g["seis"], TOL, "after errors")
+    with pytest.raises(sam.SAM2DError):
+        sam.Modeller(_grid(g, fdop=2), g["vp"], g["rho"])

@@ -1,0 +1,124 @@
+"""CPU tests of the product library's host side: the C ABI exports, and the
+FP32 coefficient fields / source increments it builds, against the FP64
+oracle's setup (which is pinned to the reference in test_oracle.py)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import seisacomod2d_amd as sam
+from conftest import ROOT, load_golden
+
+
+def declared_symbols():
+    syms = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if f.endswith(".h"):
+            txt = open(os.path.join(inc, f)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            syms |= set(re.findall(r"\b(sam2d_\w+)\s*\(", txt))
+            syms |= set(re.findall(r"\b(modelling_module|SeisAcoMod2D_\w+)\s*\(", txt))
+    return sorted(syms)
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = sam.lib()
+    syms = declared_symbols()
+    assert "sam2d_forward" in syms and "sam2d_gradient" in syms
+    missing = [s for s in syms if s != "modelling_module" and not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_version_and_variants():
+    L = sam.lib()
+    assert b"gfx950" in L.sam2d_version()
+    v = sam.variants()
+    assert len(v) == L.sam2d_num_variants() >= 4
+    assert all(name.startswith("waves") for name in v)
+
+
+def _grid_from(g, npml=32):
+    return sam.Grid(g["nx"], g["nz"], g["dx"], g["dz"], g["rec_len"], g["fd_dt"], g["real_dt"],
+                    g["fm"], npml)
+
+
+@pytest.mark.parametrize("case,npml", [("hetero_explicit", 32), ("rect_dtfactor", 32),
+                                       ("salt_small", 40), ("tiny_twolayer", 7)])
+def test_coefficients_match_oracle_setup(oracle, case, npml):
+    g = load_golden(case)
+    grid = _grid_from(g, npml)
+    c = sam.coefficients(grid, g["vp"], g["rho"])
+    P = oracle.params(g["nx"], g["nz"], g["dx"], g["dz"], g["rec_len"], g["fd_dt"], g["real_dt"],
+                      g["fm"], npml)
+    o = oracle.setup(P, g["vp"], g["rho"])
+    # Profiles are FP32 in the reference: identical.
+    np.testing.assert_array_equal(c["prof_x"][0], o["damp2a"])
+    np.testing.assert_array_equal(c["prof_x"][1], o["damp2a1"])
+    np.testing.assert_array_equal(c["prof_x"][2], o["damp2b"])
+    np.testing.assert_array_equal(c["prof_x"][3], o["damp2b1"])
+    np.testing.assert_array_equal(c["prof_z"][0], o["damp1a"])
+    np.testing.assert_array_equal(c["prof_z"][1], o["damp1a1"])
+    np.testing.assert_array_equal(c["prof_z"][2], o["damp1b"])
+    np.testing.assert_array_equal(c["prof_z"][3], o["damp1b1"])
+    # The kernels form kappaw2 = KX * damp2a1[ix] etc.  Where the profile is 1
+    # (interior) this is the reference's FP64 coefficient rounded once to FP32,
+    # bit for bit; in the sponge it is within 2 float ulps.
+    px1, pz1 = c["prof_x"][1][:, None], c["prof_z"][1][None, :]
+    bx1, bz1 = c["prof_x"][3][:, None], c["prof_z"][3][None, :]
+    for ours, ref in [(c["KX"] * px1, o["kappaw2"]), (c["KZ"] * pz1, o["kappaw1"]),
+                      (c["BX"] * bx1, o["buw2"]), (c["BZ"] * bz1, o["buw1"])]:
+        r32 = ref.astype(np.float32)
+        inner = (px1 == 1) & (pz1 == 1) & (bx1 == 1) & (bz1 == 1)
+        inner = np.broadcast_to(inner, ref.shape)
+        np.testing.assert_array_equal(ours[inner], r32[inner])
+        np.testing.assert_allclose(ours, ref, rtol=2.5e-7, atol=0)
+
+
+def test_source_increments_match_reference_formula(oracle):
+    g = load_golden("hetero_explicit")
+    grid = _grid_from(g)
+    sx, sz = g["src_xz"][0]
+    amp = sam.source_increments(grid, g["vp"], g["rho"], sx, sz)
+    P = oracle.params(g["nx"], g["nz"], g["dx"], g["dz"], g["rec_len"], g["fd_dt"], g["real_dt"],
+                      g["fm"], 32)
+    o = oracle.setup(P, g["vp"], g["rho"])
+    f = np.float32
+    ix, iz = int(f(sx) * (f(1) / f(grid.dx))), int(f(sz) * (f(1) / f(grid.dz)))
+    k = o["kappa"][ix + 32, iz + 32]
+    expect = (np.float64(f(grid.fd_dt)) * o["wlt"] * k / np.float64(f(grid.dx) * f(grid.dz))).astype(np.float32)
+    np.testing.assert_array_equal(amp, expect)
+
+
+def test_windowed_coefficients_are_the_cropped_model(oracle):
+    g = load_golden("salt_small")
+    grid = _grid_from(g)
+    wx0, w = 37, 90
+    c = sam.coefficients(grid, g["vp"], g["rho"], wx0, w)
+    grid_w = sam.Grid(w, g["nz"], g["dx"], g["dz"], g["rec_len"], g["fd_dt"], g["real_dt"], g["fm"], 32)
+    cw = sam.coefficients(grid_w, g["vp"][wx0:wx0 + w], g["rho"][wx0:wx0 + w])
+    for k in c:
+        np.testing.assert_array_equal(c[k], cw[k])
+
+
+def test_bad_arguments_return_status_not_abort():
+    g = sam.Grid(10, 10, 10, 10, 0.1, 0.001, 0.001, 10)
+    vp = np.full((10, 10), 1500, np.float32)
+    with pytest.raises(sam.SAM2DError) as e:
+        sam.coefficients(g, vp, vp, 5, 10)
+    assert e.value.code == -6
+    L = sam.lib()
+    gc = g._c()
+    gc.dx = 10.0
+    # CFL: dt > 0.606 h / vmax -> -5 (check_stability.cpp:30-43)
+    gc.fd_dt = 0.0041
+    assert L.sam2d_check_stability(ctypes.byref(gc), 1500.0, 1500.0) == -5
+    gc.fd_dt = 0.004
+    assert L.sam2d_check_stability(ctypes.byref(gc), 1500.0, 1500.0) == 0
+    # dispersion: 2 f > vmin / (5 hmax) -> -5; exactly at the limit passes
+    gc.dom_freq = 15.0
+    assert L.sam2d_check_stability(ctypes.byref(gc), 1500.0, 1500.0) == 0
+    gc.dom_freq = 15.01
+    assert L.sam2d_check_stability(ctypes.byref(gc), 1500.0, 1500.0) == -5
