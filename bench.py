@@ -100,17 +100,26 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="profiling runs only: shorten each shot to this many time steps")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="profiling calibration: one 1 GiB device-to-device copy, then exit")
+    ap.add_argument("--tuning", action="store_true", help="print per-variant autotune times")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on a single GPU; --gpus must be 1 "
               "(shots are independent, multi-GPU scaling is a shot queue per device).",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import seisacomod2d_amd as sam
 
     vp, rho = model()
-    grid = sam.Grid(NX, NZ, H, H, T_REC, DT, DT, FM, npml=NPML)
+    t_rec = T_REC if args.profile_steps <= 0 else args.profile_steps * DT
+    grid = sam.Grid(NX, NZ, H, H, t_rec, DT, DT, FM, npml=NPML)
     all_shots = shots(sam)
     hip = _Hip()
     with sam.Modeller(grid, vp, rho) as m:
@@ -121,6 +130,9 @@ def main():
         else:
             best = m.autotune(steps=6)
             print(f"# autotune picked {sam.variants()[best]}", file=sys.stderr, flush=True)
+            if args.tuning:
+                for name, ms in zip(sam.variants(), m.tuning()):
+                    print(f"#   {name:24s} {ms * 1e3:8.1f} us/step", file=sys.stderr)
         ng = all_shots[0].nrec
         per_shot = ng * m.real_nt
         seis = hip.malloc(per_shot * 4)
@@ -175,6 +187,21 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(vp, rho)
     print(json.dumps(out), flush=True)
+
+
+def copy_probe(nbytes=1 << 30):
+    """A plain device-to-device copy of known size (a blit kernel), so that
+    rocprofv3's FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
+    hip = _Hip()
+    a, b = hip.malloc(nbytes), hip.malloc(nbytes)
+    for _ in range(3):
+        rc = hip.L.hipMemcpy(ctypes.c_void_p(b), ctypes.c_void_p(a), ctypes.c_size_t(nbytes), 3)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+    hip.sync()
+    hip.free(a)
+    hip.free(b)
+    print(json.dumps({"copy_probe_bytes": nbytes, "repeats": 3}))
 
 
 def _measured_traffic():

@@ -157,6 +157,10 @@ int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value);
 int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps,
                    int *best_variant);
 
+/* Milliseconds per step the last sam2d_autotune measured for each variant
+ * (0 for variants not timed); fills min(n, sam2d_num_variants()) values. */
+int sam2d_get_tuning(const sam2d_ctx *ctx, double *ms_per_step, int n);
+
 /* Number of kernel variants and a printable description of one. */
 int sam2d_num_variants(void);
 const char *sam2d_variant_name(int variant);

@@ -174,3 +174,70 @@ def time_steps(P, vp, rho, nsteps):
     th = ctypes.c_int()
     t = L.orc_time_steps(ctypes.byref(P), _ptr(vp), _ptr(rho), int(nsteps), ctypes.byref(th))
     return t, th.value
+
+
+def dminus_T(g):
+    g = np.ascontiguousarray(g, np.float64)
+    out = np.zeros_like(g)
+    lib().orc_dminus_T(_ptr(g, ctypes.c_double), ctypes.c_int(g.size), _ptr(out, ctypes.c_double))
+    return out
+
+
+def dplus_T(g):
+    g = np.ascontiguousarray(g, np.float64)
+    out = np.zeros_like(g)
+    lib().orc_dplus_T(_ptr(g, ctypes.c_double), ctypes.c_int(g.size), _ptr(out, ctypes.c_double))
+    return out
+
+
+def _geom(src_xz, rec_xz_list, adj_list=None):
+    src = _f32(src_xz).reshape(-1, 2)
+    nrec = np.array([len(r) for r in rec_xz_list], np.int32)
+    rec = _f32(np.concatenate([np.asarray(r, np.float32).reshape(-1, 2) for r in rec_xz_list]))
+    adj = None
+    if adj_list is not None:
+        adj = _f32(np.concatenate([np.asarray(a, np.float32).ravel() for a in adj_list]))
+    return src, nrec, rec, adj
+
+
+def functional(P, vp, rho, src_xz, rec_xz_list, adj_list):
+    """J = sum adj * p at the recorded samples (FP64 pressure, double sum)."""
+    vp, rho = _f32(vp), _f32(rho)
+    src, nrec, rec, adj = _geom(src_xz, rec_xz_list, adj_list)
+    L = lib()
+    L.orc_functional.restype = ctypes.c_double
+    L.orc_functional.argtypes = [ctypes.POINTER(OrcParams)] + [ctypes.POINTER(ctypes.c_float)] * 2 + [
+        ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int),
+        ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    return L.orc_functional(ctypes.byref(P), _ptr(vp), _ptr(rho), src.shape[0], _ptr(src),
+                            _ptr(nrec, ctypes.c_int), _ptr(rec), _ptr(adj))
+
+
+def gradient(P, vp, rho, src_xz, rec_xz_list, adj_list, padded=False):
+    """FP64 dJ/dvp, dJ/drho (nx, nz) of J = sum adj * p; with padded=True also
+    the padded-grid dJ/dkappa, dJ/dbu1, dJ/dbu2."""
+    fd_nt, real_nt, dtf, nts, nnx, nnz = shape(P)
+    vp, rho = _f32(vp), _f32(rho)
+    src, nrec, rec, adj = _geom(src_xz, rec_xz_list, adj_list)
+    gvp = np.zeros((P.nx, P.nz), np.float64)
+    grho = np.zeros_like(gvp)
+    gk, gb1, gb2 = (np.zeros((nnx, nnz), np.float64) for _ in range(3))
+    L = lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.orc_gradient.argtypes = [ctypes.POINTER(OrcParams)] + [ctypes.POINTER(ctypes.c_float)] * 2 + [
+        ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int),
+        ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)] + [dp] * 5
+    rc = L.orc_gradient(ctypes.byref(P), _ptr(vp), _ptr(rho), src.shape[0], _ptr(src),
+                        _ptr(nrec, ctypes.c_int), _ptr(rec), _ptr(adj), _ptr(gvp, ctypes.c_double),
+                        _ptr(grho, ctypes.c_double), _ptr(gk, ctypes.c_double),
+                        _ptr(gb1, ctypes.c_double), _ptr(gb2, ctypes.c_double))
+    if rc != 0:
+        raise RuntimeError(f"orc_gradient failed: {rc}")
+    if padded:
+        return gvp, grho, dict(kappa=gk, bu1=gb1, bu2=gb2)
+    return gvp, grho
+
+
+def set_f64_buoyancy(on):
+    """Test-only: evaluate 1/rho and the staggered buoyancy averages in FP64."""
+    lib().orc_set_f64_buoyancy(ctypes.c_int(1 if on else 0))

@@ -225,6 +225,8 @@ static void orc_model_free(orc_model *m)
     free(m->wlt);
 }
 
+static int orc_f64_buoyancy = 0;
+
 /* modelling_module() setup, cpumodelling.cpp:168-281: expand vp/rho, stagger,
  * Ricker, sponge profiles (fac = 0.05f, :254) and the four scaled
  * coefficient fields (:270-281). */
@@ -261,16 +263,43 @@ static void orc_model_init(orc_model *m, int nx1, int nz1, float dx, float dz,
     orc_pml_b(fac, npml, nz1, m->damp1b, m->damp1b1);
     m->kappaw1 = (double *)calloc(N, 8); m->kappaw2 = (double *)calloc(N, 8);
     m->buw1 = (double *)calloc(N, 8); m->buw2 = (double *)calloc(N, 8);
+    double *b1 = NULL, *b2 = NULL;
+    if (orc_f64_buoyancy) {
+        /* test-only: the same staggering rules with b, bu1, bu2 in FP64 */
+        double *b = (double *)calloc(N, 8);
+        b1 = (double *)calloc(N, 8); b2 = (double *)calloc(N, 8);
+        for (long i = 0; i < N; i++) b[i] = rhoe[i] > 0.0f ? 1.0 / (double)rhoe[i] : 1e10;
+        for (int ix = 0; ix < nnx; ix++)
+            for (int iz = 0; iz + 1 < nnz; iz++) {
+                long id = iz + (long)ix * nnz;
+                b1[id] = 0.5 * (b[id + 1] + b[id]);
+            }
+        for (int ix = 0; ix + 1 < nnx; ix++)
+            for (int iz = 0; iz < nnz; iz++) {
+                long id = iz + (long)ix * nnz;
+                b2[id] = 0.5 * (b[id + nnz] + b[id]);
+            }
+        for (int iz = 0; iz < nnz; iz++)
+            b2[(long)(nnx - 1) * nnz + iz] = b1[(long)(nnx - 2) * nnz + iz];
+        free(b);
+    }
     for (int ix = 0; ix < nnx; ix++)
         for (int iz = 0; iz < nnz; iz++) {
             long id = iz + (long)ix * nnz;
             m->kappaw2[id] = m->dtdx * m->kappa[id] * m->damp2a1[ix];
             m->kappaw1[id] = m->dtdz * m->kappa[id] * m->damp1a1[iz];
-            m->buw2[id] = m->dtdx * bu2[id] * m->damp2b1[ix];
-            m->buw1[id] = m->dtdz * bu1[id] * m->damp1b1[iz];
+            m->buw2[id] = m->dtdx * (b2 ? b2[id] : bu2[id]) * m->damp2b1[ix];
+            m->buw1[id] = m->dtdz * (b1 ? b1[id] : bu1[id]) * m->damp1b1[iz];
         }
+    free(b1); free(b2);
     free(vpe); free(rhoe); free(bue); free(bu1); free(bu2);
 }
+
+/* Test-only switch: evaluate the buoyancy chain (1/rho and the staggered
+ * averages) in FP64 instead of the reference's FP32, so that finite
+ * differences in rho are not swamped by float rounding of b.  The adjoint
+ * is the same for both. */
+void orc_set_f64_buoyancy(int on) { orc_f64_buoyancy = on; }
 
 /* Grid index of a coordinate in metres, cpumodelling.cpp:288-289, 333-334
  * with cpu_set_sg() (cpu_modelling_kernels.cpp:15-22): float multiply by the
@@ -468,4 +497,232 @@ double orc_time_steps(const orc_params *P, const float *vp, const float *rho, in
     orc_state_free(&s);
     orc_model_free(&m);
     return t;
+}
+
+/* =================================================================== FWI */
+/* The reference has no adjoint (job_t.adj is declared and never used,
+ * src/src_cpu/modelling.h:51).  What follows is the exact discrete adjoint of
+ * the FP64 forward above (orc_step), derived operator by operator, for a
+ * linear functional of the recorded pressure:
+ *
+ *   J(vp, rho) = sum over shots, receivers g, samples k of adj[g][k] * p_k[G_g]
+ *
+ * where p_k is the FP64 pressure the forward samples (before its cast to
+ * float).  For a misfit L(d) pass adj = dL/dd.  The forward states are kept
+ * in full (small grids only).  It is pinned by central finite differences of
+ * orc_functional and by the transposition identities in the tests. */
+
+/* Transposes written as scatters of the forward stencils (so they share no
+ * code with any gather form): out[k] += w(i, k) * g[i] for every tap of
+ * D-(i) / D+(i) on a line of n points with stride s. */
+static void dminus_T_scatter(const double *g, double *out, int n, long s, long base)
+{
+    for (int i = 0; i < n; i++) {
+        const double gi = g[base + i * s];
+        if (gi == 0.0) continue;
+        if (i >= 2 && i <= n - 2) {
+            out[base + i * s] += ORC_A0 * gi;
+            out[base + (i - 1) * s] -= ORC_A0 * gi;
+            out[base + (i + 1) * s] += ORC_A1 * gi;
+            out[base + (i - 2) * s] -= ORC_A1 * gi;
+        } else if (i == 1 || i == n - 1) {
+            out[base + i * s] += gi;
+            out[base + (i - 1) * s] -= gi;
+        }
+    }
+}
+static void dplus_T_scatter(const double *g, double *out, int n, long s, long base)
+{
+    for (int i = 0; i < n; i++) {
+        const double gi = g[base + i * s];
+        if (gi == 0.0) continue;
+        if (i >= 1 && i <= n - 3) {
+            out[base + (i + 1) * s] += ORC_A0 * gi;
+            out[base + i * s] -= ORC_A0 * gi;
+            out[base + (i + 2) * s] += ORC_A1 * gi;
+            out[base + (i - 1) * s] -= ORC_A1 * gi;
+        } else if (i == 0 || i == n - 2) {
+            out[base + (i + 1) * s] += gi;
+            out[base + i * s] -= gi;
+        }
+    }
+}
+
+/* Line operators exported for the transposition tests. */
+void orc_dminus_T(const double *g, int n, double *out)
+{
+    memset(out, 0, sizeof(double) * n);
+    dminus_T_scatter(g, out, n, 1, 0);
+}
+void orc_dplus_T(const double *g, int n, double *out)
+{
+    memset(out, 0, sizeof(double) * n);
+    dplus_T_scatter(g, out, n, 1, 0);
+}
+
+/* J for one model (double accumulation of the FP64 samples). */
+double orc_functional(const orc_params *P, const float *vp, const float *rho, int ns,
+                      const float *src_xz, const int *nrec, const float *rec_xz,
+                      const float *adj)
+{
+    orc_model m;
+    orc_model_init(&m, P->nx, P->nz, P->dx, P->dz, vp, rho, P->rec_len, P->fd_dt,
+                   P->real_dt, P->fm, P->npml);
+    orc_state s;
+    orc_state_alloc(&s, m.N);
+    double J = 0.0;
+    long rec_off = 0, a_off = 0;
+    for (int is = 0; is < ns; is++) {
+        const int ng = nrec[is];
+        long isrc = orc_cell(&m, src_xz[2 * is], src_xz[2 * is + 1]);
+        for (long i = 0; i < m.N; i++) { s.p[i] = s.px[i] = s.pz[i] = s.vx[i] = s.vz[i] = 0.0; }
+        for (int kt = 0; kt < m.fd_nt; kt++) {
+            orc_step(&m, &s, kt, isrc, 1.0);
+            if ((kt + 1) % m.dt_factor == 0) {
+                const int k = (kt + 1) / m.dt_factor - 1;
+                if (k < m.real_nt)
+                    for (int g = 0; g < ng; g++) {
+                        long gc = orc_cell(&m, rec_xz[2 * (rec_off + g)], rec_xz[2 * (rec_off + g) + 1]);
+                        J += (double)adj[a_off + (long)g * m.real_nt + k] * s.p[gc];
+                    }
+            }
+        }
+        rec_off += ng;
+        a_off += (long)ng * m.real_nt;
+    }
+    orc_state_free(&s);
+    orc_model_free(&m);
+    return J;
+}
+
+/* dJ/dvp and dJ/drho on the model grid (x-major nx*nz, FP64), plus the
+ * intermediate gradients on the padded grid when the pointers are given:
+ * g_kappa (dJ/dkappa), g_bu1, g_bu2 (dJ/d of the staggered buoyancies). */
+int orc_gradient(const orc_params *P, const float *vp, const float *rho, int ns,
+                 const float *src_xz, const int *nrec, const float *rec_xz, const float *adj,
+                 double *g_vp, double *g_rho, double *gk_out, double *gb1_out, double *gb2_out)
+{
+    orc_model m;
+    orc_model_init(&m, P->nx, P->nz, P->dx, P->dz, vp, rho, P->rec_len, P->fd_dt,
+                   P->real_dt, P->fm, P->npml);
+    const int nnz = m.nnz, nnx = m.nnx, nt = m.fd_nt;
+    const long N = m.N;
+    double *gk = (double *)calloc(N, 8), *gb1 = (double *)calloc(N, 8), *gb2 = (double *)calloc(N, 8);
+    /* stored forward fields: vx, vz before step n and p after the p-phase of n */
+    double *Vx = (double *)malloc(sizeof(double) * N * nt);
+    double *Vz = (double *)malloc(sizeof(double) * N * nt);
+    double *Pn = (double *)malloc(sizeof(double) * N * nt);
+    double *lpx = (double *)calloc(N, 8), *lpz = (double *)calloc(N, 8);
+    double *lvx = (double *)calloc(N, 8), *lvz = (double *)calloc(N, 8);
+    double *lp = (double *)calloc(N, 8), *t1 = (double *)calloc(N, 8), *t2 = (double *)calloc(N, 8);
+    double *nvx = (double *)calloc(N, 8), *nvz = (double *)calloc(N, 8);
+    if (!Vx || !Vz || !Pn) return -30;
+    orc_state s;
+    orc_state_alloc(&s, m.N);
+    long rec_off = 0, a_off = 0;
+    for (int is = 0; is < ns; is++) {
+        const int ng = nrec[is];
+        const long isrc = orc_cell(&m, src_xz[2 * is], src_xz[2 * is + 1]);
+        long *gc = (long *)malloc(sizeof(long) * (ng > 0 ? ng : 1));
+        for (int g = 0; g < ng; g++)
+            gc[g] = orc_cell(&m, rec_xz[2 * (rec_off + g)], rec_xz[2 * (rec_off + g) + 1]);
+        for (long i = 0; i < N; i++) { s.p[i] = s.px[i] = s.pz[i] = s.vx[i] = s.vz[i] = 0.0; }
+        for (int n = 0; n < nt; n++) {
+            memcpy(Vx + (long)n * N, s.vx, sizeof(double) * N);
+            memcpy(Vz + (long)n * N, s.vz, sizeof(double) * N);
+            orc_step(&m, &s, n, isrc, 1.0);
+            memcpy(Pn + (long)n * N, s.p, sizeof(double) * N);
+        }
+        for (long i = 0; i < N; i++) { lpx[i] = lpz[i] = lvx[i] = lvz[i] = 0.0; }
+        for (int n = nt - 1; n >= 0; n--) {
+            const double *vxo = Vx + (long)n * N, *vzo = Vz + (long)n * N, *pn = Pn + (long)n * N;
+            /* record (cpu_record after step n) */
+            for (long i = 0; i < N; i++) lp[i] = 0.0;
+            if ((n + 1) % m.dt_factor == 0) {
+                const int k = (n + 1) / m.dt_factor - 1;
+                if (k < m.real_nt)
+                    for (int g = 0; g < ng; g++) lp[gc[g]] += adj[a_off + (long)g * m.real_nt + k];
+            }
+            /* v-update: vx' = damp2b vx + buw2 Dx+ p ;  vz' = damp1b vz + buw1 Dz+ p */
+            for (int ix = 0; ix < nnx; ix++)
+                for (int iz = 0; iz < nnz; iz++) {
+                    const long id = iz + (long)ix * nnz;
+                    t1[id] = m.buw2[id] * lvx[id];
+                    t2[id] = m.buw1[id] * lvz[id];
+                    gb2[id] += lvx[id] * dplus_at(pn + id, ix, nnx, nnz) * m.dtdx * m.damp2b1[ix];
+                    gb1[id] += lvz[id] * dplus_at(pn + id, iz, nnz, 1) * m.dtdz * m.damp1b1[iz];
+                    nvx[id] = m.damp2b[ix] * lvx[id];
+                    nvz[id] = m.damp1b[iz] * lvz[id];
+                }
+            for (int iz = 0; iz < nnz; iz++) dplus_T_scatter(t1, lp, nnx, nnz, iz);
+            for (int ix = 0; ix < nnx; ix++) dplus_T_scatter(t2, lp, nnz, 1, (long)ix * nnz);
+            /* p = px + pz, then the source on px */
+            for (long i = 0; i < N; i++) { lpx[i] += lp[i]; lpz[i] += lp[i]; }
+            if (n < m.nts) gk[isrc] += lpx[isrc] * (m.fd_dt * m.wlt[n] / (m.dx * m.dz));
+            /* p-update: px' = damp2a px + kappaw2 Dx- vx ;  pz' = damp1a pz + kappaw1 Dz- vz */
+            for (int ix = 0; ix < nnx; ix++)
+                for (int iz = 0; iz < nnz; iz++) {
+                    const long id = iz + (long)ix * nnz;
+                    gk[id] += lpx[id] * dminus_at(vxo + id, ix, nnx, nnz) * m.dtdx * m.damp2a1[ix]
+                            + lpz[id] * dminus_at(vzo + id, iz, nnz, 1) * m.dtdz * m.damp1a1[iz];
+                    t1[id] = m.kappaw2[id] * lpx[id];
+                    t2[id] = m.kappaw1[id] * lpz[id];
+                    lpx[id] *= m.damp2a[ix];
+                    lpz[id] *= m.damp1a[iz];
+                }
+            for (int iz = 0; iz < nnz; iz++) dminus_T_scatter(t1, nvx, nnx, nnz, iz);
+            for (int ix = 0; ix < nnx; ix++) dminus_T_scatter(t2, nvz, nnz, 1, (long)ix * nnz);
+            memcpy(lvx, nvx, sizeof(double) * N);
+            memcpy(lvz, nvz, sizeof(double) * N);
+        }
+        free(gc);
+        rec_off += ng;
+        a_off += (long)ng * m.real_nt;
+    }
+    if (gk_out) memcpy(gk_out, gk, sizeof(double) * N);
+    if (gb1_out) memcpy(gb1_out, gb1, sizeof(double) * N);
+    if (gb2_out) memcpy(gb2_out, gb2, sizeof(double) * N);
+
+    /* Chain to the padded vp, rho (cpu_stagger, cpu_modelling_kernels.cpp:64-121):
+     * bu2's last column copies bu1's column nnx-2; bu1's last row is constant 0;
+     * bu1 = (b[iz] + b[iz+1])/2, bu2 = (b[ix] + b[ix+1])/2, b = 1/rho;
+     * kappa = vp^2 rho. */
+    float *vpe = (float *)calloc(N, 4), *rhoe = (float *)calloc(N, 4);
+    orc_expand(vp, P->nx, P->nz, P->npml, vpe);
+    orc_expand(rho, P->nx, P->nz, P->npml, rhoe);
+    for (int iz = 0; iz < nnz; iz++)
+        gb1[(long)(nnx - 2) * nnz + iz] += gb2[(long)(nnx - 1) * nnz + iz];
+    double *gb = (double *)calloc(N, 8);
+    for (int ix = 0; ix < nnx; ix++)
+        for (int iz = 0; iz + 1 < nnz; iz++) {
+            const long id = iz + (long)ix * nnz;
+            gb[id] += 0.5 * gb1[id];
+            gb[id + 1] += 0.5 * gb1[id];
+        }
+    for (int ix = 0; ix + 1 < nnx; ix++)
+        for (int iz = 0; iz < nnz; iz++) {
+            const long id = iz + (long)ix * nnz;
+            gb[id] += 0.5 * gb2[id];
+            gb[id + nnz] += 0.5 * gb2[id];
+        }
+    for (long i = 0; i < (long)P->nx * P->nz; i++) { g_vp[i] = 0.0; g_rho[i] = 0.0; }
+    for (int ix = 0; ix < nnx; ix++)
+        for (int iz = 0; iz < nnz; iz++) {
+            const long id = iz + (long)ix * nnz;
+            const double v = vpe[id], r = rhoe[id];
+            const double gv = 2.0 * v * r * gk[id];
+            double gr = v * v * gk[id];
+            if (rhoe[id] > 0.0f) gr += -gb[id] / (r * r);
+            int mx = ix - P->npml, mz = iz - P->npml;
+            mx = mx < 0 ? 0 : (mx >= P->nx ? P->nx - 1 : mx);
+            mz = mz < 0 ? 0 : (mz >= P->nz ? P->nz - 1 : mz);
+            g_vp[(long)mx * P->nz + mz] += gv;
+            g_rho[(long)mx * P->nz + mz] += gr;
+        }
+    free(vpe); free(rhoe); free(gb);
+    free(gk); free(gb1); free(gb2); free(Vx); free(Vz); free(Pn);
+    free(lpx); free(lpz); free(lvx); free(lvz); free(lp); free(t1); free(t2); free(nvx); free(nvz);
+    orc_state_free(&s);
+    orc_model_free(&m);
+    return 0;
 }

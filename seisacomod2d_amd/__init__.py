@@ -264,6 +264,14 @@ class Modeller:
             return gvp, grho, self._split(seis, shots)
         return gvp, grho
 
+    def tuning(self):
+        """ms per step of every variant from the last autotune (0 = not timed)."""
+        n = lib().sam2d_num_variants()
+        out = (ctypes.c_double * n)()
+        lib().sam2d_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        _check(lib().sam2d_get_tuning(self._h, out, n))
+        return list(out)
+
     def autotune(self, windows=None, nshots=1, steps=8):
         best = ctypes.c_int(-1)
         w, wp = _windows_c(windows, nshots) if windows is not None else (None, None)

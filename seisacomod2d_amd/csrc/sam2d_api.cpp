@@ -675,6 +675,14 @@ int sam2d_source_increments(const sam2d_grid *g, const float *vp, const float *r
     return SAM2D_OK;
 }
 
+int sam2d_get_tuning(const sam2d_ctx *ctx, double *ms_per_step, int n)
+{
+    if (!ctx || !ms_per_step) return fail(SAM2D_ERR_PARAM, "NULL argument");
+    const int m = std::min(n, std::min(num_variants(), 64));
+    for (int i = 0; i < m; i++) ms_per_step[i] = ctx->tune_ms[i];
+    return SAM2D_OK;
+}
+
 int sam2d_get_stats(const sam2d_ctx *ctx, sam2d_stats *out)
 {
     if (!ctx || !out) return fail(SAM2D_ERR_PARAM, "NULL argument");

@@ -47,6 +47,14 @@ __device__ __forceinline__ void dplus_coef(int i, int n, float &c0, float &c1)
     else { c0 = 0.0f; c1 = 0.0f; }
 }
 
+// Descriptor pointers are generic; casting them into the global address space
+// makes the compiler emit global_* instead of flat_* memory instructions.
+// Flat accesses also count against lgkmcnt, so every lane-shuffle wait would
+// otherwise drain all loads in flight and serialise the column prefetch.
+#define SAM2D_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ SAM2D_GLOBAL T *gp(T *p) { return (SAM2D_GLOBAL T *)p; }
+
 __device__ __forceinline__ float shfl_up(float v, int d) { return __shfl_up(v, d, 64); }
 __device__ __forceinline__ float shfl_dn(float v, int d) { return __shfl_down(v, d, 64); }
 
@@ -78,23 +86,55 @@ __device__ __forceinline__ void gather_receivers(const ShotDev *sh, int nshots, 
 {
     for (int s = 0; s < nshots; ++s) {
         const ShotDev &d = sh[s];
-        const float *px = d.st[par][0];
-        const float *pz = d.st[par][1];
+        const SAM2D_GLOBAL float *px = gp(d.st[par][0]);
+        const SAM2D_GLOBAL float *pz = gp(d.st[par][1]);
+        const SAM2D_GLOBAL int *gc = gp(d.gcell);
+        SAM2D_GLOBAL float *seis = gp(d.seis);
         for (int g = gtid; g < d.ng; g += gstride) {
-            int c = d.gcell[g];
-            d.seis[(size_t)g * d.real_nt + slot] = px[c] + pz[c];
+            int c = gc[g];
+            seis[(size_t)g * d.real_nt + slot] = px[c] + pz[c];
         }
     }
+}
+
+// Operands of one column that the march loads (one row per lane).
+struct ColOps {
+    float vx_next;              // vx at column c+1
+    float vz, px, pz, kx, kz;   // at column c
+    float dax, skx;             // x-profiles at column c (uniform)
+    float bx, bz, vz_j;         // BX, BZ and vz at column j = c-2
+    float dbx, sbx;             // x-profiles at column j (uniform)
+};
+
+template <bool NT>
+__device__ __forceinline__ void st(SAM2D_GLOBAL float *p, unsigned i, float v)
+{
+    if (NT) __builtin_nontemporal_store(v, p + i);
+    else p[i] = v;
 }
 
 // One time step.  par selects the input buffer set (output is 1 - par).
 // rec_slot >= 0 asks for the receiver gather of the state held in the input
 // buffers (the previous step's result) into sample rec_slot.
-template <int WAVES>
+//
+// WAVES: z-strips per workgroup.  PF: columns of operands in flight (2 or 4).
+// NTS: non-temporal hint on the state stores.
+//
+// The march along x is unrolled by four columns so that the sliding windows
+// (vx at c-2..c+1, new p at c-3..c) rotate by compile-time index and the
+// operand slots are refilled in place: no register a load is still writing
+// is ever copied, which keeps PF columns of loads in flight.
+//
+// Loads use clamped indices and are never masked: a lane outside the grid
+// (halo rows above row 0 or below nnz-1, columns outside [0, nnx)) reads a
+// finite in-grid value, and every such value meets a zero stencil coefficient
+// (the reference's edge rules) before it could reach an owned cell.
+template <int WAVES, int PF, bool NTS>
 __global__ __launch_bounds__(WAVES * 64)
 void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
             int ntiles, int step, int par, int rec_slot)
 {
+    static_assert(PF == 1 || PF == 2 || PF == 4, "PF must divide the unroll of 4");
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
 
@@ -117,24 +157,23 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         const int xb = min(xa + d.chunk, nnx);
 
         const int r = zb - HALO_T + lane;             // this lane's row
-        const bool rvalid = (r >= 0) && (r < nnz);
         const bool rown = (lane >= HALO_T) && (lane < HALO_T + OWN_Z) && (r < nnz);
-        const int rc = min(max(r, 0), nnz - 1);
+        const unsigned rc = (unsigned)min(max(r, 0), nnz - 1);
 
-        const float *__restrict__ pxi = d.st[par][0];
-        const float *__restrict__ pzi = d.st[par][1];
-        const float *__restrict__ vxi = d.st[par][2];
-        const float *__restrict__ vzi = d.st[par][3];
-        float *__restrict__ pxo = d.st[par ^ 1][0];
-        float *__restrict__ pzo = d.st[par ^ 1][1];
-        float *__restrict__ vxo = d.st[par ^ 1][2];
-        float *__restrict__ vzo = d.st[par ^ 1][3];
-        const float *__restrict__ KX = d.KX;
-        const float *__restrict__ KZ = d.KZ;
-        const float *__restrict__ BX = d.BX;
-        const float *__restrict__ BZ = d.BZ;
-        const float *__restrict__ prx = d.prof_x;     // [4][nnx]
-        const float *__restrict__ prz = d.prof_z;     // [4][nnz]
+        const SAM2D_GLOBAL float *__restrict__ pxi = gp(d.st[par][0]);
+        const SAM2D_GLOBAL float *__restrict__ pzi = gp(d.st[par][1]);
+        const SAM2D_GLOBAL float *__restrict__ vxi = gp(d.st[par][2]);
+        const SAM2D_GLOBAL float *__restrict__ vzi = gp(d.st[par][3]);
+        SAM2D_GLOBAL float *__restrict__ pxo = gp(d.st[par ^ 1][0]);
+        SAM2D_GLOBAL float *__restrict__ pzo = gp(d.st[par ^ 1][1]);
+        SAM2D_GLOBAL float *__restrict__ vxo = gp(d.st[par ^ 1][2]);
+        SAM2D_GLOBAL float *__restrict__ vzo = gp(d.st[par ^ 1][3]);
+        const SAM2D_GLOBAL float *__restrict__ KX = gp(d.KX);
+        const SAM2D_GLOBAL float *__restrict__ KZ = gp(d.KZ);
+        const SAM2D_GLOBAL float *__restrict__ BX = gp(d.BX);
+        const SAM2D_GLOBAL float *__restrict__ BZ = gp(d.BZ);
+        const SAM2D_GLOBAL float *__restrict__ prx = gp(d.prof_x);   // [4][nnx]
+        const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);   // [4][nnz]
 
         // Per-row constants: sponge profiles and z-operator coefficients.
         const float daz = prz[rc], skz = prz[nnz + rc];
@@ -143,75 +182,88 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         dminus_coef(r, nnz, zm0, zm1);
         dplus_coef(r, nnz, zp0, zp1);
         const bool src_row = (r == d.src_z);
-        const float amp = (step < d.nts) ? d.amp[step] : 0.0f;
+        const float amp = (step < d.nts) ? gp(d.amp)[step] : 0.0f;
         const int src_x = (step <= d.nts) ? d.src_x : -1;
 
-        auto ld = [&](const float *__restrict__ f, int col) -> float {
-            return (rvalid && col >= 0 && col < nnx) ? f[(size_t)col * nnz + r] : 0.0f;
+        // Column base (uniform, in elements) of a clamped column index.
+        auto colbase = [&](int col) -> unsigned {
+            return (unsigned)min(max(col, 0), nnx - 1) * (unsigned)nnz;
+        };
+        auto load_col = [&](int c) -> ColOps {
+            ColOps o;
+            const unsigned b1 = colbase(c + 1), b0 = colbase(c), bj = colbase(c - 2);
+            o.vx_next = vxi[b1 + rc];
+            o.vz = vzi[b0 + rc]; o.px = pxi[b0 + rc]; o.pz = pzi[b0 + rc];
+            o.kx = KX[b0 + rc]; o.kz = KZ[b0 + rc];
+            o.bx = BX[bj + rc]; o.bz = BZ[bj + rc]; o.vz_j = vzi[bj + rc];
+            const int cc = min(max(c, 0), nnx - 1), jc = min(max(c - 2, 0), nnx - 1);
+            o.dax = prx[cc]; o.skx = prx[nnx + cc];
+            o.dbx = prx[2 * nnx + jc]; o.sbx = prx[3 * nnx + jc];
+            return o;
         };
 
-        // Sliding windows: vx at c-2, c-1, c; vz at c-1, c-2; p at c-3..c-1.
-        float vx_m2 = ld(vxi, xa - 3), vx_m1 = ld(vxi, xa - 2), vx_0 = ld(vxi, xa - 1);
-        float vz_m1 = 0.0f, vz_m2 = 0.0f;
-        float p_m3 = 0.0f, p_m2 = 0.0f, p_m1 = 0.0f;
+        // Windows, indexed modulo 4 relative to the unrolled block's base
+        // column c: vxw[k & 3] = vx at c-2+k; pw[k & 3] = new p at c-3+k.
+        float vxw[4], pw[4];
+        vxw[0] = vxi[colbase(xa - 3) + rc];
+        vxw[1] = vxi[colbase(xa - 2) + rc];
+        vxw[2] = vxi[colbase(xa - 1) + rc];
+        vxw[3] = 0.0f;
+        pw[0] = pw[1] = pw[2] = pw[3] = 0.0f;
 
-        // Prefetched operands of the first column.
-        int c = xa - 1;
-        float n_vx = ld(vxi, c + 1), n_vz = ld(vzi, c), n_px = ld(pxi, c), n_pz = ld(pzi, c);
-        float n_kx = ld(KX, c), n_kz = ld(KZ, c), n_bx = ld(BX, c - 2), n_bz = ld(BZ, c - 2);
+        ColOps q[PF];
+        #pragma unroll
+        for (int u = 0; u < PF; u++) q[u] = load_col(xa - 1 + u);
 
-        for (; c <= xb + 1; ++c) {
-            const float vx_p1 = n_vx, vz_0 = n_vz, px0 = n_px, pz0 = n_pz;
-            const float kx = n_kx, kz = n_kz, bx = n_bx, bz = n_bz;
-            if (c + 1 <= xb + 1) {
-                n_vx = ld(vxi, c + 2); n_vz = ld(vzi, c + 1);
-                n_px = ld(pxi, c + 1); n_pz = ld(pzi, c + 1);
-                n_kx = ld(KX, c + 1); n_kz = ld(KZ, c + 1);
-                n_bx = ld(BX, c - 1); n_bz = ld(BZ, c - 1);
-            }
-
-            // ---- p-phase at column c (cpu_pml_p, :335-351; source, :135-140)
-            const bool cvalid = (c >= 0) && (c < nnx);
-            float dax = 0.0f, skx = 0.0f;
-            if (cvalid) { dax = prx[c]; skx = prx[nnx + c]; }
-            float xm0, xm1;
-            dminus_coef(c, nnx, xm0, xm1);
-            const float dvx = fmaf(xm0, vx_0 - vx_m1, xm1 * (vx_p1 - vx_m2));
-            const float vz_u2 = shfl_up(vz_0, 2), vz_u1 = shfl_up(vz_0, 1);
-            const float vz_d1 = shfl_dn(vz_0, 1);
-            const float dvz = fmaf(zm0, vz_0 - vz_u1, zm1 * (vz_d1 - vz_u2));
-            float pxn = fmaf(kx * skx, dvx, dax * px0);
-            const float pzn = fmaf(kz * skz, dvz, daz * pz0);
-            if (src_row && c == src_x) pxn += amp;
-            const float pn = pxn + pzn;
-            if (rown && c >= xa && c < xb) {
-                const size_t id = (size_t)c * nnz + r;
-                pxo[id] = pxn;
-                pzo[id] = pzn;
-            }
-
-            // ---- v-phase at column j = c - 2 (cpu_pml_v, :353-368)
-            const int j = c - 2;
-            if (j >= xa && j < xb) {
-                const float dbx = prx[2 * nnx + j], sbx = prx[3 * nnx + j];
-                float xp0, xp1;
-                dplus_coef(j, nnx, xp0, xp1);
-                const float dpx = fmaf(xp0, p_m1 - p_m2, xp1 * (pn - p_m3));
-                const float q_u1 = shfl_up(p_m2, 1);
-                const float q_d1 = shfl_dn(p_m2, 1), q_d2 = shfl_dn(p_m2, 2);
-                const float dpz = fmaf(zp0, q_d1 - p_m2, zp1 * (q_d2 - q_u1));
-                const float vxn = fmaf(bx * sbx, dpx, dbx * vx_m2);
-                const float vzn = fmaf(bz * sbz, dpz, dbz * vz_m2);
-                if (rown) {
-                    const size_t id = (size_t)j * nnz + r;
-                    vxo[id] = vxn;
-                    vzo[id] = vzn;
+        // The march may run up to three columns past xb+1: those columns store
+        // nothing (stores need c < xb, j < xb).
+        for (int c0 = xa - 1; c0 <= xb + 1; c0 += 4) {
+            #pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int c = c0 + u;
+                ColOps &cur = q[u % PF];
+                // ---- p-phase at column c (cpu_pml_p, :335-351; source, :135-140)
+                vxw[(u + 3) & 3] = cur.vx_next;
+                const float vx_m2 = vxw[u & 3], vx_m1 = vxw[(u + 1) & 3];
+                const float vx_0 = vxw[(u + 2) & 3], vx_p1 = vxw[(u + 3) & 3];
+                float xm0, xm1;
+                dminus_coef(c, nnx, xm0, xm1);
+                const float dvx = fmaf(xm0, vx_0 - vx_m1, xm1 * (vx_p1 - vx_m2));
+                const float vz_0 = cur.vz;
+                const float vz_u2 = shfl_up(vz_0, 2), vz_u1 = shfl_up(vz_0, 1);
+                const float vz_d1 = shfl_dn(vz_0, 1);
+                const float dvz = fmaf(zm0, vz_0 - vz_u1, zm1 * (vz_d1 - vz_u2));
+                float pxn = fmaf(cur.kx * cur.skx, dvx, cur.dax * cur.px);
+                const float pzn = fmaf(cur.kz * skz, dvz, daz * cur.pz);
+                if (src_row && c == src_x) pxn += amp;
+                const float pn = pxn + pzn;
+                pw[(u + 3) & 3] = pn;
+                if (rown && c >= xa && c < xb) {
+                    const unsigned id = (unsigned)c * (unsigned)nnz + (unsigned)r;
+                    st<NTS>(pxo, id, pxn);
+                    st<NTS>(pzo, id, pzn);
                 }
-            }
 
-            vx_m2 = vx_m1; vx_m1 = vx_0; vx_0 = vx_p1;
-            vz_m2 = vz_m1; vz_m1 = vz_0;
-            p_m3 = p_m2; p_m2 = p_m1; p_m1 = pn;
+                // ---- v-phase at column j = c - 2 (cpu_pml_v, :353-368)
+                const int j = c - 2;
+                if (j >= xa && j < xb) {
+                    const float p_m3 = pw[u & 3], p_m2 = pw[(u + 1) & 3], p_m1 = pw[(u + 2) & 3];
+                    float xp0, xp1;
+                    dplus_coef(j, nnx, xp0, xp1);
+                    const float dpx = fmaf(xp0, p_m1 - p_m2, xp1 * (pn - p_m3));
+                    const float q_u1 = shfl_up(p_m2, 1);
+                    const float q_d1 = shfl_dn(p_m2, 1), q_d2 = shfl_dn(p_m2, 2);
+                    const float dpz = fmaf(zp0, q_d1 - p_m2, zp1 * (q_d2 - q_u1));
+                    const float vxn = fmaf(cur.bx * cur.sbx, dpx, cur.dbx * vx_m2);
+                    const float vzn = fmaf(cur.bz * sbz, dpz, dbz * cur.vz_j);
+                    if (rown) {
+                        const unsigned id = (unsigned)j * (unsigned)nnz + (unsigned)r;
+                        st<NTS>(vxo, id, vxn);
+                        st<NTS>(vzo, id, vzn);
+                    }
+                }
+                cur = load_col(c + PF);
+            }
         }
     }
 }
@@ -235,30 +287,67 @@ __global__ void k_fill(float *__restrict__ p, size_t n, float v)
 // ---------------------------------------------------------------- launchers
 
 static const int kWaves[] = {1, 2, 4, 8};
-static const int kChunks[] = {32, 64, 128, 256, 512};
+static const int kChunks[] = {16, 32, 64, 128, 256};
+static const int kPF[] = {2, 4};
+static const int kNTS[] = {0, 1};
+static const int nW = 4, nC = 5, nP = 2, nN = 2;
 
-int num_variants() { return (int)(sizeof(kWaves) / sizeof(int) * sizeof(kChunks) / sizeof(int)); }
+int num_variants() { return nW * nC * nP * nN; }
+
+// Variant index -> (waves, chunk, prefetch, nt-stores); chunk varies fastest.
+static void decode(int v, int &w, int &c, int &pf, int &nts)
+{
+    c = kChunks[v % nC]; v /= nC;
+    w = kWaves[v % nW]; v /= nW;
+    pf = kPF[v % nP]; v /= nP;
+    nts = kNTS[v % nN];
+}
 
 void variant_shape(int v, int *waves, int *chunk)
 {
-    const int nc = (int)(sizeof(kChunks) / sizeof(int));
-    *waves = kWaves[v / nc];
-    *chunk = kChunks[v % nc];
+    int pf, nts;
+    decode(v, *waves, *chunk, pf, nts);
 }
 
-hipError_t launch_step(int waves, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
+void variant_name(int v, char *buf, int n)
+{
+    int w, c, pf, nts;
+    decode(v, w, c, pf, nts);
+    snprintf(buf, n, "waves%d_chunk%d_pf%d%s", w, c, pf, nts ? "_nts" : "");
+}
+
+template <int W, int PF, bool NTS>
+static void launch_t(dim3 g, hipStream_t s, const ShotDev *d, int ns, int vt, int per, int nt,
+                     int step, int par, int slot)
+{
+    hipLaunchKernelGGL((k_step<W, PF, NTS>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step, par, slot);
+}
+
+template <int W>
+static void launch_w(int pf, int nts, dim3 g, hipStream_t s, const ShotDev *d, int ns, int vt,
+                     int per, int nt, int step, int par, int slot)
+{
+    if (pf == 2 && !nts) launch_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, par, slot);
+    else if (pf == 2) launch_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, par, slot);
+    else if (!nts) launch_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, par, slot);
+    else launch_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, par, slot);
+}
+
+hipError_t launch_step(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
                        int step, int par, int rec_slot, hipStream_t stream)
 {
+    int waves, chunk, pf, nts;
+    decode(variant, waves, chunk, pf, nts);
     const int per = (ntiles + 7) / 8;
     const int vtotal = per * 8;
     int grid = vtotal;
     if (grid_cap > 0 && grid > grid_cap) grid = (grid_cap + 7) / 8 * 8;
-    dim3 block(64 * waves);
+    const dim3 g(grid);
     switch (waves) {
-    case 1: hipLaunchKernelGGL(k_step<1>, dim3(grid), block, 0, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
-    case 2: hipLaunchKernelGGL(k_step<2>, dim3(grid), block, 0, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
-    case 4: hipLaunchKernelGGL(k_step<4>, dim3(grid), block, 0, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
-    case 8: hipLaunchKernelGGL(k_step<8>, dim3(grid), block, 0, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
+    case 1: launch_w<1>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
+    case 2: launch_w<2>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
+    case 4: launch_w<4>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
+    case 8: launch_w<8>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -285,11 +374,10 @@ hipError_t launch_fill(float *p, size_t n, float v, hipStream_t stream)
 }  // namespace sam2d
 
 namespace sam2d {
-// Variant used until the autotuner has run: 4 strips per workgroup, 128-column
-// chunks.
+// Variant used until the autotuner has run: 4 strips per workgroup, 32-column
+// chunks, two columns of operands in flight, ordinary stores.
 int default_variant_index()
 {
-    const int nc = (int)(sizeof(kChunks) / sizeof(int));
-    return 2 * nc + 2;
+    return ((0 * nP + 0) * nW + 2) * nC + 1;
 }
 }  // namespace sam2d
