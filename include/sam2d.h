@@ -147,7 +147,9 @@ int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
 /* Tuning and options.  Keys: "graph" (0/1, default 1), "variant" (-1 = auto,
  * else a kernel variant index), "reset_quirk" (1 = zero only the first nnz
  * cells between shots, cpumodelling.cpp:343-348; default 0 = fresh state),
- * "max_batch" (shots resident at once in batched calls, 0 = all). */
+ * "max_batch" (shots resident at once in batched and gradient calls, 0 = all
+ * or, for gradients, as many as fit "hbm_budget_gb" (default 64 GiB of device
+ * memory for ring, checkpoints and adjoint state)). */
 int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value);
 
 /* Project-written autotuner: times every kernel variant (z-strips per

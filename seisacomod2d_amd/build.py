@@ -3,7 +3,9 @@
     python -m seisacomod2d_amd.build        (from the repository root)
 
 Produces
-  seisacomod2d_amd/libsam2d.so            the C ABI of include/sam2d.h (HIP kernels + host)
+  seisacomod2d_amd/libsam2d.so             the C ABI of include/sam2d.h (HIP kernels + host)
+  seisacomod2d_amd/libsam2d_host.so        include/sam2d_modelling.h: job card, geometry,
+                                           model files, modelling_module drop-in
   seisacomod2d_amd/SeisAcoMod2D_MI355X.exe the job-card driver (reference CLI equivalent)
 """
 import os
@@ -71,12 +73,15 @@ def build(force=True, jobs=8):
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, lib)
 
-    host_src = [os.path.join(HOST, s) for s in HOST_SOURCES if os.path.exists(os.path.join(HOST, s))]
-    if host_src:
-        exe = os.path.join(PKG, "SeisAcoMod2D_MI355X.exe")
-        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
-              "-I" + HOST, "-o", exe] + host_src + [os.path.join(HOST, "main.cpp"),
-              "-L" + PKG, "-lsam2d", "-Wl,-rpath,$ORIGIN"])
+    host_src = [os.path.join(HOST, s) for s in HOST_SOURCES]
+    hlib = os.path.join(PKG, "libsam2d_host.so")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-I" + os.path.join(ROOT, "include"),
+          "-o", hlib + ".tmp"] + host_src + ["-L" + PKG, "-lsam2d", "-Wl,-rpath,$ORIGIN"])
+    os.replace(hlib + ".tmp", hlib)
+    exe = os.path.join(PKG, "SeisAcoMod2D_MI355X.exe")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-o", exe,
+          os.path.join(HOST, "main.cpp"), "-L" + PKG, "-lsam2d_host", "-lsam2d",
+          "-Wl,-rpath,$ORIGIN"])
     return lib
 
 

@@ -1,15 +1,311 @@
-// sam2d_adjoint.cpp -- FWI gradient entry point (filled in with the adjoint kernels).
+// sam2d_adjoint.cpp -- sam2d_gradient: checkpointed FWI gradient.
+//
+// Per batch of shots:
+//   1. forward over all fd_nt steps in two ring slots, saving the state that
+//      enters step m*k into checkpoint m (and the seismograms if asked for);
+//   2. backward, segment by segment from the last: restore checkpoint m into
+//      ring slot 0, recompute steps [m*k, m*k + k) into ring slots 1..k
+//      (bit-identical to the first pass: same kernel, same inputs), then run
+//      the adjoint steps of the segment in reverse, each reading the forward
+//      states it needs from the ring;
+//   3. copy dJ/dkappa, dJ/dbu2, dJ/dbu1 to the host and chain them in FP64 to
+//      dJ/dvp and dJ/drho through the buoyancy staggering, kappa = vp^2 rho and
+//      the sponge padding (the transpose of cpu_stagger,
+//      src/src_cpu/cpu_modelling_kernels.cpp:64-121, and expand,
+//      src/src_cpu/cpumodelling.cpp:39-60).
 #include "sam2d_internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace sam2d {
+
+struct AdjShotBuffers {
+    float *lam = nullptr;     // 2 adjoint slots x 4 fields, then gk, gbx, gbz
+    float *ckpt = nullptr;    // nckpt slots x 4 fields
+    float *srcg = nullptr;
+    int *colptr = nullptr, *rrow = nullptr, *rid = nullptr;
+    float *res = nullptr;
+    size_t cap[7] = {0, 0, 0, 0, 0, 0, 0};   // bytes of each buffer above
+    ~AdjShotBuffers()
+    {
+        dev_free(lam); dev_free(ckpt); dev_free(srcg);
+        dev_free(colptr); dev_free(rrow); dev_free(rid); dev_free(res);
+    }
+};
+
+struct AdjointState {
+    std::vector<std::unique_ptr<AdjShotBuffers>> bufs;
+    AdjDev *d_desc = nullptr;
+    int cap_desc = 0;
+    ~AdjointState() { dev_free(d_desc); }
+};
+
+static int grow(void **p, size_t &cap, size_t need)
+{
+    if (need <= cap) return SAM2D_OK;
+    dev_free(*p);
+    *p = nullptr;
+    cap = 0;
+    SAM2D_TRY(dev_alloc(p, need));
+    cap = need;
+    return SAM2D_OK;
+}
+
+// Receiver-sample index recorded after step n, or -1.
+static int rec_sample(const TimeAxis &ta, int n)
+{
+    if ((n + 1) % ta.dt_factor != 0) return -1;
+    const int k = (n + 1) / ta.dt_factor - 1;
+    return k < ta.real_nt ? k : -1;
+}
+
+// Host chain rule of one shot's padded-grid gradients into the model grid.
+static void chain_to_model(const sam2d_ctx *ctx, const DomainHost &h, const std::vector<float> &gk,
+                           const std::vector<float> &gbx, const std::vector<float> &gbz,
+                           std::vector<double> &g_vp, std::vector<double> &g_rho)
+{
+    const int nnz = h.nnz, nnx = h.nnx, P = ctx->grid.npml, nz = ctx->grid.nz;
+    const size_t N = (size_t)nnz * nnx;
+    std::vector<double> b1(gbz.begin(), gbz.end()), b2(gbx.begin(), gbx.end());
+    // bu2's last column is bu1's column nnx-2.
+    for (int iz = 0; iz < nnz; iz++) b1[(size_t)(nnx - 2) * nnz + iz] += b2[(size_t)(nnx - 1) * nnz + iz];
+    // bu1 = (b[iz] + b[iz+1]) / 2 except the last row (a constant 0);
+    // bu2 = (b[ix] + b[ix+1]) / 2 except the last column (handled above).
+    std::vector<double> gb(N, 0.0);
+    for (int ix = 0; ix < nnx; ix++)
+        for (int iz = 0; iz + 1 < nnz; iz++) {
+            const size_t id = (size_t)ix * nnz + iz;
+            gb[id] += 0.5 * b1[id];
+            gb[id + 1] += 0.5 * b1[id];
+        }
+    for (int ix = 0; ix + 1 < nnx; ix++)
+        for (int iz = 0; iz < nnz; iz++) {
+            const size_t id = (size_t)ix * nnz + iz;
+            gb[id] += 0.5 * b2[id];
+            gb[id + nnz] += 0.5 * b2[id];
+        }
+    for (int ix = 0; ix < nnx; ix++) {
+        int mx = ix - P;
+        mx = mx < 0 ? 0 : (mx >= h.w ? h.w - 1 : mx);
+        const size_t col = (size_t)(h.wx0 + mx) * nz;
+        for (int iz = 0; iz < nnz; iz++) {
+            int mz = iz - P;
+            mz = mz < 0 ? 0 : (mz >= nz ? nz - 1 : mz);
+            const size_t id = (size_t)ix * nnz + iz;
+            const double v = ctx->vp[col + mz], r = ctx->rho[col + mz];
+            g_vp[col + mz] += 2.0 * v * r * (double)gk[id];
+            double gr = v * v * (double)gk[id];
+            if (ctx->rho[col + mz] > 0.0f) gr -= gb[id] / (r * r);
+            g_rho[col + mz] += gr;
+        }
+    }
+}
+
+// Device bytes one shot of `plan` needs for a gradient with checkpoint interval k.
+static size_t shot_bytes(const ShotPlan &p, int fd_nt, int k)
+{
+    const size_t N = (size_t)p.dom->host.nnx * p.dom->host.nnz;
+    const size_t nck = (size_t)(fd_nt + k - 1) / k;
+    return N * 4 * (size_t)(k + 1 + nck + 2) * sizeof(float) + 3 * N * sizeof(float);
+}
+
+static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const float *const *adj,
+                          int k, float *const *seis_out, std::vector<double> &g_vp,
+                          std::vector<double> &g_rho)
+{
+    AdjointState &A = *ctx->adj;
+    const TimeAxis &ta = ctx->ta;
+    const int ns = (int)plans.size();
+    const int variant = pick_variant(ctx);
+    int ntiles = 0;
+    SAM2D_TRY(stage_batch(ctx, plans, variant, &ntiles, k + 1));
+    if ((int)A.bufs.size() < ns) A.bufs.resize(ns);
+    std::vector<AdjDev> desc(ns);
+    const int nseg = (ta.fd_nt + k - 1) / k;
+    for (int s = 0; s < ns; s++) {
+        ShotPlan &p = plans[s];
+        const DomainHost &h = p.dom->host;
+        const size_t N = (size_t)h.nnx * h.nnz;
+        if (!A.bufs[s]) A.bufs[s].reset(new AdjShotBuffers());
+        AdjShotBuffers &b = *A.bufs[s];
+        const size_t ng1 = (size_t)std::max(1, p.ng);
+        SAM2D_TRY(grow((void **)&b.lam, b.cap[0], 11 * N * sizeof(float)));
+        SAM2D_TRY(grow((void **)&b.ckpt, b.cap[1], (size_t)nseg * 4 * N * sizeof(float)));
+        SAM2D_TRY(grow((void **)&b.srcg, b.cap[2], (size_t)std::max(1, ta.nts) * sizeof(float)));
+        SAM2D_TRY(grow((void **)&b.colptr, b.cap[3], (size_t)(h.nnx + 1) * sizeof(int)));
+        SAM2D_TRY(grow((void **)&b.rrow, b.cap[4], ng1 * sizeof(int)));
+        SAM2D_TRY(grow((void **)&b.rid, b.cap[5], ng1 * sizeof(int)));
+        SAM2D_TRY(grow((void **)&b.res, b.cap[6], ng1 * ta.real_nt * sizeof(float)));
+        float *lam = b.lam, *grad = b.lam + 8 * N;
+        HIP_TRY(launch_fill(lam, 11 * N, 0.0f, ctx->stream));
+
+        // Source-weight per step and receivers by column (CSR).
+        std::vector<float> srcg(std::max(1, ta.nts), 0.0f);
+        const double area = (double)(ctx->grid.dx * ctx->grid.dz);
+        for (int n = 0; n < ta.nts; n++)
+            srcg[n] = (float)((double)ctx->grid.fd_dt * ctx->wavelet[n] / area);
+        std::vector<int> colptr(h.nnx + 1, 0), rrow(std::max(1, p.ng)), rid(std::max(1, p.ng));
+        for (int g = 0; g < p.ng; g++) colptr[p.gcell[g] / h.nnz + 1]++;
+        for (int c = 0; c < h.nnx; c++) colptr[c + 1] += colptr[c];
+        std::vector<int> fillp(colptr.begin(), colptr.end() - 1);
+        for (int g = 0; g < p.ng; g++) {
+            const int c = p.gcell[g] / h.nnz;
+            const int e = fillp[c]++;
+            rrow[e] = p.gcell[g] % h.nnz;
+            rid[e] = g;
+        }
+        SAM2D_TRY(upload(b.srcg, srcg.data(), srcg.size() * 4, ctx->stream));
+        SAM2D_TRY(upload(b.colptr, colptr.data(), colptr.size() * 4, ctx->stream));
+        SAM2D_TRY(upload(b.rrow, rrow.data(), (size_t)p.ng * 4, ctx->stream));
+        SAM2D_TRY(upload(b.rid, rid.data(), (size_t)p.ng * 4, ctx->stream));
+        SAM2D_TRY(upload(b.res, adj[s], (size_t)p.ng * ta.real_nt * 4, ctx->stream));
+
+        const ShotDev &fd = ctx->staged_desc[s];
+        AdjDev &d = desc[s];
+        std::memset(&d, 0, sizeof d);
+        d.nnz = fd.nnz; d.nnx = fd.nnx; d.ngroups = fd.ngroups; d.nchunks = fd.nchunks;
+        d.chunk = fd.chunk; d.tile_begin = fd.tile_begin;
+        d.src_x = p.src_x; d.src_z = p.src_z; d.nts = ta.nts; d.real_nt = ta.real_nt;
+        d.dtdx = (float)ta.dtdx; d.dtdz = (float)ta.dtdz;
+        d.srcg = b.srcg;
+        d.KX = fd.KX; d.KZ = fd.KZ; d.BX = fd.BX; d.BZ = fd.BZ;
+        d.prof_x = fd.prof_x; d.prof_z = fd.prof_z;
+        d.fwd = fd.state; d.lam = lam; d.ncell = N;
+        d.gk = grad; d.gbx = grad + N; d.gbz = grad + 2 * N;
+        d.colptr = b.colptr; d.rrow = b.rrow; d.rid = b.rid; d.res = b.res;
+    }
+    if (ns > A.cap_desc) {
+        dev_free(A.d_desc); A.d_desc = nullptr; A.cap_desc = 0;
+        SAM2D_TRY(dev_alloc((void **)&A.d_desc, sizeof(AdjDev) * ns));
+        A.cap_desc = ns;
+    }
+    SAM2D_TRY(upload(A.d_desc, desc.data(), sizeof(AdjDev) * ns, ctx->stream));
+    int max_ng = 0;
+    for (auto &p : plans) max_ng = std::max(max_ng, p.ng);
+
+    auto save_or_restore = [&](int m, int slot, bool save) -> int {
+        for (int s = 0; s < ns; s++) {
+            ShotBuffers &fb = *ctx->bufs[s];
+            const size_t bytes = 4 * fb.cells * sizeof(float);
+            float *ck = A.bufs[s]->ckpt + (size_t)m * 4 * fb.cells;
+            HIP_TRY(hipMemcpyAsync(save ? ck : fb.slot(slot), save ? fb.slot(slot) : ck, bytes,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        return SAM2D_OK;
+    };
+
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    // 1. forward with checkpoints
+    const bool want_seis = seis_out != nullptr;
+    for (int n = 0; n < ta.fd_nt; n++) {
+        if (n % k == 0) SAM2D_TRY(save_or_restore(n / k, n & 1, true));
+        int rs = -1;
+        if (want_seis && n >= 1 && n % ta.dt_factor == 0 && n / ta.dt_factor - 1 < ta.real_nt)
+            rs = n / ta.dt_factor - 1;
+        HIP_TRY(launch_step(variant, ctx->d_desc, ns, ntiles, ctx->grid_cap, n, n & 1, (n & 1) ^ 1,
+                            rs, ctx->stream));
+    }
+    if (want_seis && ta.fd_nt % ta.dt_factor == 0 && ta.fd_nt / ta.dt_factor - 1 < ta.real_nt && max_ng)
+        HIP_TRY(launch_record(ctx->d_desc, ns, max_ng, ta.fd_nt & 1, ta.fd_nt / ta.dt_factor - 1,
+                              ctx->stream));
+    // 2. backward by segments
+    int lam = 0;
+    for (int m = nseg - 1; m >= 0; m--) {
+        const int n0 = m * k, n1 = std::min(ta.fd_nt, n0 + k);
+        SAM2D_TRY(save_or_restore(m, 0, false));
+        for (int n = n0; n < n1; n++)
+            HIP_TRY(launch_step(variant, ctx->d_desc, ns, ntiles, ctx->grid_cap, n, n - n0,
+                                n - n0 + 1, -1, ctx->stream));
+        for (int n = n1 - 1; n >= n0; n--) {
+            HIP_TRY(launch_adjoint_step(variant, A.d_desc, ns, ntiles, n, lam, lam ^ 1, n - n0,
+                                        n - n0 + 1, rec_sample(ta, n), ctx->stream));
+            lam ^= 1;
+        }
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.kernel_ms += ms;
+    ctx->stats.launches += (long long)ta.fd_nt * 3;
+    ctx->stats.variant = variant;
+
+    // 3. gradients to the host, chain rule
+    for (int s = 0; s < ns; s++) {
+        const DomainHost &h = plans[s].dom->host;
+        const size_t N = (size_t)h.nnx * h.nnz;
+        ctx->stats.cell_updates += (double)N * ta.fd_nt;
+        std::vector<float> g(3 * N);
+        HIP_TRY(hipMemcpy(g.data(), A.bufs[s]->lam + 8 * N, 3 * N * sizeof(float),
+                          hipMemcpyDeviceToHost));
+        std::vector<float> gk(g.begin(), g.begin() + N), gbx(g.begin() + N, g.begin() + 2 * N),
+            gbz(g.begin() + 2 * N, g.end());
+        chain_to_model(ctx, h, gk, gbx, gbz, g_vp, g_rho);
+        if (want_seis && plans[s].ng)
+            HIP_TRY(hipMemcpy(seis_out[s], plans[s].seis_dev,
+                              (size_t)plans[s].ng * ta.real_nt * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return SAM2D_OK;
+}
+
+}  // namespace sam2d
 
 using namespace sam2d;
 
-void sam2d_adjoint_release(sam2d_ctx *ctx) { (void)ctx; }
+void sam2d_adjoint_release(sam2d_ctx *ctx)
+{
+    delete ctx->adj;
+    ctx->adj = nullptr;
+}
 
 extern "C" int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
                               const int *windows, const float *adj, int ckpt_k, float *seis,
                               float *g_vp, float *g_rho)
 {
-    (void)ctx; (void)shots; (void)nshots; (void)windows; (void)adj; (void)ckpt_k;
-    (void)seis; (void)g_vp; (void)g_rho;
-    return fail(SAM2D_ERR_PARAM, "gradient not built yet");
+    clear_error();
+    if (!ctx) return fail(SAM2D_ERR_PARAM, "ctx is NULL");
+    if (!g_vp || !g_rho) return fail(SAM2D_ERR_PARAM, "gradient outputs are NULL");
+    const TimeAxis &ta = ctx->ta;
+    size_t total = 0;
+    for (int s = 0; s < nshots; s++) total += (size_t)shots[s].nrec * ta.real_nt;
+    if (total > 0 && !adj) return fail(SAM2D_ERR_PARAM, "adj is NULL");
+    const double t0 = now_ms();
+    ctx->stats = sam2d_stats();
+    if (!ctx->adj) ctx->adj = new AdjointState();
+    int k = ckpt_k < 1 ? 1 : (ckpt_k > ta.fd_nt ? ta.fd_nt : ckpt_k);
+
+    std::vector<ShotPlan> plans;
+    SAM2D_TRY(make_plans(ctx, shots, nshots, windows, plans));
+    std::vector<const float *> adj_ptr(nshots);
+    std::vector<float *> seis_ptr(nshots);
+    size_t off = 0;
+    for (int s = 0; s < nshots; s++) {
+        adj_ptr[s] = adj + off;
+        seis_ptr[s] = seis ? seis + off : nullptr;
+        off += (size_t)shots[s].nrec * ta.real_nt;
+    }
+    // Shots per batch: the option, else as many as fit the HBM budget.
+    int batch = ctx->max_batch > 0 ? ctx->max_batch : nshots;
+    if (ctx->max_batch <= 0 && nshots > 0) {
+        const size_t budget = (size_t)ctx->hbm_budget_gb << 30;
+        size_t used = 0;
+        int fit = 0;
+        for (int s = 0; s < nshots; s++) {
+            used += shot_bytes(plans[s], ta.fd_nt, k);
+            if (used > budget && fit > 0) break;
+            fit++;
+        }
+        batch = std::max(1, fit);
+    }
+    std::vector<double> gv((size_t)ctx->grid.nx * ctx->grid.nz, 0.0), gr(gv.size(), 0.0);
+    for (int s0 = 0; s0 < nshots; s0 += batch) {
+        const int s1 = std::min(nshots, s0 + batch);
+        std::vector<ShotPlan> sub(plans.begin() + s0, plans.begin() + s1);
+        SAM2D_TRY(gradient_batch(ctx, sub, adj_ptr.data() + s0, k,
+                                 seis ? seis_ptr.data() + s0 : nullptr, gv, gr));
+    }
+    for (size_t i = 0; i < gv.size(); i++) { g_vp[i] = (float)gv[i]; g_rho[i] = (float)gr[i]; }
+    ctx->stats.wall_ms = now_ms() - t0;
+    return SAM2D_OK;
 }

@@ -74,23 +74,20 @@ ShotBuffers::~ShotBuffers() { release(); }
 
 void ShotBuffers::release()
 {
-    for (int b = 0; b < 2; b++)
-        for (int f = 0; f < 4; f++) { dev_free(st[b][f]); st[b][f] = nullptr; }
-    dev_free(amp); dev_free(gcell); dev_free(seis);
-    amp = nullptr; gcell = nullptr; seis = nullptr;
-    cap_cells = 0; cap_nts = 0; cap_ng = 0; cap_seis = 0;
+    dev_free(state); dev_free(amp); dev_free(gcell); dev_free(seis);
+    state = nullptr; amp = nullptr; gcell = nullptr; seis = nullptr;
+    cells = 0; cap_state = 0; cap_nts = 0; cap_ng = 0; cap_seis = 0;
 }
 
-int ShotBuffers::reserve(size_t cells, int nts, int ng, size_t nseis)
+int ShotBuffers::reserve(size_t ncells, int nslots, int nts, int ng, size_t nseis)
 {
-    if (cells > cap_cells) {
-        for (int b = 0; b < 2; b++)
-            for (int f = 0; f < 4; f++) {
-                dev_free(st[b][f]); st[b][f] = nullptr;
-                SAM2D_TRY(dev_alloc((void **)&st[b][f], cells * sizeof(float)));
-            }
-        cap_cells = cells;
+    const size_t need = ncells * 4 * (size_t)nslots;
+    if (need > cap_state) {
+        dev_free(state); state = nullptr; cap_state = 0;
+        SAM2D_TRY(dev_alloc((void **)&state, need * sizeof(float)));
+        cap_state = need;
     }
+    cells = ncells;
     if (nts > cap_nts) {
         dev_free(amp); amp = nullptr;
         SAM2D_TRY(dev_alloc((void **)&amp, (size_t)nts * sizeof(float)));
@@ -191,7 +188,8 @@ int pick_variant(sam2d_ctx *ctx)
 }
 
 // Make `plans` resident: buffers, descriptors, zeroed state and seismograms.
-int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *ntiles_out)
+int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *ntiles_out,
+                int nslots)
 {
     int waves, chunk;
     variant_shape(variant, &waves, &chunk);
@@ -207,10 +205,10 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
         const size_t N = (size_t)nnz * nnx;
         if (!ctx->bufs[s]) ctx->bufs[s].reset(new ShotBuffers());
         ShotBuffers &b = *ctx->bufs[s];
-        SAM2D_TRY(b.reserve(N, nts, p.ng, (size_t)p.ng * real_nt));
+        SAM2D_TRY(b.reserve(N, nslots, nts, p.ng, (size_t)p.ng * real_nt));
         SAM2D_TRY(upload(b.amp, p.amp.data(), sizeof(float) * nts, ctx->stream));
         SAM2D_TRY(upload(b.gcell, p.gcell.data(), sizeof(int) * p.ng, ctx->stream));
-        for (int f = 0; f < 4; f++) HIP_TRY(launch_fill(b.st[0][f], N, 0.0f, ctx->stream));
+        HIP_TRY(launch_fill(b.slot(0), 4 * N, 0.0f, ctx->stream));
         if (p.ng) HIP_TRY(launch_fill(b.seis, (size_t)p.ng * real_nt, 0.0f, ctx->stream));
         ShotDev &d = desc[s];
         std::memset(&d, 0, sizeof d);
@@ -224,8 +222,8 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
         d.amp = b.amp;
         d.KX = dom->KX; d.KZ = dom->KZ; d.BX = dom->BX; d.BZ = dom->BZ;
         d.prof_x = dom->prof_x; d.prof_z = dom->prof_z;
-        for (int bb = 0; bb < 2; bb++)
-            for (int f = 0; f < 4; f++) d.st[bb][f] = b.st[bb][f];
+        d.state = b.state;
+        d.ncell = N;
         d.gcell = b.gcell; d.seis = b.seis;
         p.seis_dev = b.seis;
     }
@@ -254,11 +252,11 @@ static int rec_slot_for_kernel(const TimeAxis &ta, int n)
 }
 
 // Enqueue steps [n0, n1) of the time loop (the tail record when n1 == fd_nt).
-int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int waves, int n0, int n1, int max_ng)
+int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int n0, int n1, int max_ng)
 {
     for (int n = n0; n < n1; n++)
-        HIP_TRY(launch_step(waves, ctx->d_desc, nshots, ntiles, ctx->grid_cap, n, n & 1,
-                            rec_slot_for_kernel(ctx->ta, n), ctx->stream));
+        HIP_TRY(launch_step(variant, ctx->d_desc, nshots, ntiles, ctx->grid_cap, n, n & 1,
+                            (n & 1) ^ 1, rec_slot_for_kernel(ctx->ta, n), ctx->stream));
     if (n1 == ctx->ta.fd_nt) {
         const int slot = rec_slot_for_kernel(ctx->ta, n1);
         if (slot >= 0 && max_ng > 0)
@@ -284,7 +282,7 @@ int run_time_loop(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int max_n
         if (!exec) {
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_steps(ctx, nshots, ntiles, waves, 0, fd_nt, max_ng);
+            int rc = enqueue_steps(ctx, nshots, ntiles, variant, 0, fd_nt, max_ng);
             hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
             if (rc != SAM2D_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
             HIP_TRY(e);
@@ -303,7 +301,7 @@ int run_time_loop(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int max_n
         HIP_TRY(hipGraphLaunch(exec, ctx->stream));
         used_graph = true;
     } else {
-        SAM2D_TRY(enqueue_steps(ctx, nshots, ntiles, waves, 0, fd_nt, max_ng));
+        SAM2D_TRY(enqueue_steps(ctx, nshots, ntiles, variant, 0, fd_nt, max_ng));
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev1));
@@ -366,11 +364,9 @@ static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, flo
         const int fin = ctx->ta.fd_nt & 1;
         if (s > 0 && b0) {
             // Bring the final state into set 0, then clear the first nnz cells.
-            for (int f = 0; f < 4; f++)
-                if (fin) HIP_TRY(hipMemcpyAsync(b0->st[0][f], b0->st[1][f], N * 4, hipMemcpyDeviceToDevice, ctx->stream));
             keep.resize(4 * N);
-            for (int f = 0; f < 4; f++)
-                HIP_TRY(hipMemcpyAsync(&keep[f * N], b0->st[0][f], N * 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(keep.data(), b0->slot(fin), 4 * N * 4, hipMemcpyDeviceToHost,
+                                   ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             const int nnz = one[0].dom->host.nnz;
             for (int f = 0; f < 4; f++) std::fill(&keep[f * N], &keep[f * N] + nnz, 0.0f);
@@ -379,12 +375,11 @@ static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, flo
         SAM2D_TRY(stage_batch(ctx, one, variant, &ntiles));
         if (s > 0 && !keep.empty()) {
             ShotBuffers &b = *ctx->bufs[0];
-            for (int f = 0; f < 4; f++)
-                SAM2D_TRY(upload(b.st[0][f], &keep[f * N], N * 4, ctx->stream));
+            SAM2D_TRY(upload(b.slot(0), keep.data(), 4 * N * 4, ctx->stream));
         }
         ctx->stats.cell_updates += (double)N * ctx->ta.fd_nt;
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        SAM2D_TRY(enqueue_steps(ctx, 1, ntiles, waves, 0, ctx->ta.fd_nt, one[0].ng));
+        SAM2D_TRY(enqueue_steps(ctx, 1, ntiles, variant, 0, ctx->ta.fd_nt, one[0].ng));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(hipEventSynchronize(ctx->ev1));
         float ms = 0;
@@ -445,9 +440,7 @@ const char *sam2d_variant_name(int v)
 {
     static thread_local char buf[64];
     if (v < 0 || v >= num_variants()) return "invalid";
-    int w, c;
-    variant_shape(v, &w, &c);
-    snprintf(buf, sizeof buf, "waves%d_chunk%d", w, c);
+    variant_name(v, buf, sizeof buf);
     return buf;
 }
 
@@ -589,6 +582,10 @@ int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value)
     } else if (k == "reset_quirk") ctx->reset_quirk = value != 0;
     else if (k == "max_batch") ctx->max_batch = value < 0 ? 0 : value;
     else if (k == "grid_cap") ctx->grid_cap = value < 0 ? 0 : value;
+    else if (k == "hbm_budget_gb") {
+        if (value < 1) return fail(SAM2D_ERR_PARAM, "hbm_budget_gb must be >= 1");
+        ctx->hbm_budget_gb = value;
+    }
     else return fail(SAM2D_ERR_PARAM, "unknown option '%s'", key);
     return SAM2D_OK;
 }
@@ -616,14 +613,15 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
     plans.resize(batch);
     double best_ms = 1e300;
     int best = default_variant_index();
+    ctx->tune_ms.assign(num_variants(), 0.0);
     for (int v = 0; v < num_variants(); v++) {
         int waves, chunk;
         variant_shape(v, &waves, &chunk);
         int ntiles = 0;
         SAM2D_TRY(stage_batch(ctx, plans, v, &ntiles));
-        SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, waves, 0, 2, 0));   // warm
+        SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, v, 0, 2, 0));   // warm
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, waves, 2, 2 + steps, 0));
+        SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, v, 2, 2 + steps, 0));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(hipEventSynchronize(ctx->ev1));
         float ms = 0;
@@ -678,8 +676,9 @@ int sam2d_source_increments(const sam2d_grid *g, const float *vp, const float *r
 int sam2d_get_tuning(const sam2d_ctx *ctx, double *ms_per_step, int n)
 {
     if (!ctx || !ms_per_step) return fail(SAM2D_ERR_PARAM, "NULL argument");
-    const int m = std::min(n, std::min(num_variants(), 64));
-    for (int i = 0; i < m; i++) ms_per_step[i] = ctx->tune_ms[i];
+    const int m = std::min(n, num_variants());
+    for (int i = 0; i < m; i++)
+        ms_per_step[i] = i < (int)ctx->tune_ms.size() ? ctx->tune_ms[i] : 0.0;
     return SAM2D_OK;
 }
 

@@ -28,17 +28,54 @@ struct ShotDev {
     const float *BX, *BZ;  // dt/dx*bu2,  dt/dz*bu1
     const float *prof_x;   // [4][nnx]: damp2a, damp2a1, damp2b, damp2b1
     const float *prof_z;   // [4][nnz]: damp1a, damp1a1, damp1b, damp1b1
-    float *st[2][4];       // [buffer set][px, pz, vx, vz]
+    float *state;          // slots of 4 fields (px, pz, vx, vz), each ncell floats:
+    size_t ncell;          //   field f of slot s at state + (4*s + f)*ncell
     const int *gcell;      // [ng] receiver cells (ix*nnz + iz)
     float *seis;           // [ng][real_nt]
 };
 
+__host__ __device__ inline float *slot_field(const ShotDev &d, int slot, int f)
+{
+    return d.state + ((size_t)slot * 4 + f) * d.ncell;
+}
+
+// One shot of the adjoint (backward) time loop.  Tiles as in ShotDev.
+struct AdjDev {
+    int nnz, nnx;
+    int ngroups, nchunks, chunk, tile_begin;
+    int src_x, src_z, nts, real_nt;
+    float dtdx, dtdz;      // fd_dt/dx, fd_dt/dz as the coefficients use them
+    const float *srcg;     // [nts] d(source increment)/d(kappa_s) = fd_dt*w[n]/(dx*dz)
+    const float *KX, *KZ, *BX, *BZ;
+    const float *prof_x, *prof_z;
+    const float *fwd;      // forward ring: slots of 4 fields (px, pz, vx, vz)
+    float *lam;            // adjoint state: 2 slots of 4 fields
+    size_t ncell;
+    float *gk;             // dJ/dkappa      (padded grid)
+    float *gbx;            // dJ/dbu2        (x-staggered buoyancy)
+    float *gbz;            // dJ/dbu1        (z-staggered buoyancy)
+    const int *colptr;     // [nnx+1] receivers of each column (CSR)
+    const int *rrow;       // [ng] padded row of each entry
+    const int *rid;        // [ng] receiver index of each entry
+    const float *res;      // [ng][real_nt] adjoint source dJ/dseis
+};
+
+// One backward step n of every staged shot: adjoint state lam_in -> lam_out,
+// forward states S(n-1) in ring slot fwd_prev and S(n) in fwd_cur; rec_k >= 0
+// injects residual sample rec_k.
+hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
+                               int step, int lam_in, int lam_out, int fwd_prev, int fwd_cur,
+                               int rec_k, hipStream_t stream);
+
 int num_variants();
 void variant_shape(int v, int *waves, int *chunk);
+void variant_name(int v, char *buf, int n);
 
-hipError_t launch_step(int waves, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
-                       int step, int par, int rec_slot, hipStream_t stream);
-hipError_t launch_record(const ShotDev *d_shots, int nshots, int max_ng, int par, int slot,
+// One time step of every staged shot: reads state slot in_slot, writes
+// out_slot; rec_sample >= 0 gathers the receivers of the in_slot state.
+hipError_t launch_step(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
+                       int step, int in_slot, int out_slot, int rec_sample, hipStream_t stream);
+hipError_t launch_record(const ShotDev *d_shots, int nshots, int max_ng, int slot, int sample,
                          hipStream_t stream);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t stream);
 

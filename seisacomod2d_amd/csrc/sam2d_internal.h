@@ -49,13 +49,14 @@ struct DeviceDomain {
 
 // Device state of one resident shot slot (grown on demand, reused).
 struct ShotBuffers {
-    float *st[2][4] = {{nullptr}};
+    float *state = nullptr;        // nslots x 4 fields x cells
     float *amp = nullptr;
     int *gcell = nullptr;
     float *seis = nullptr;
-    size_t cap_cells = 0, cap_seis = 0;
+    size_t cells = 0, cap_state = 0, cap_seis = 0;
     int cap_nts = 0, cap_ng = 0;
-    int reserve(size_t cells, int nts, int ng, size_t nseis);
+    int reserve(size_t cells, int nslots, int nts, int ng, size_t nseis);
+    float *slot(int s) const { return state + (size_t)s * 4 * cells; }
     void release();
     ~ShotBuffers();
 };
@@ -104,7 +105,8 @@ struct sam2d_ctx {
     int variant = -1, tuned_variant = -1, default_variant = 0;
     int max_batch = 0;
     int grid_cap = 0;
-    double tune_ms[64] = {0};
+    int hbm_budget_gb = 64;                // device memory a gradient batch may use
+    std::vector<double> tune_ms;           // per variant, from the autotuner
     sam2d_stats stats = sam2d_stats();
     sam2d::AdjointState *adj = nullptr;
 };
@@ -115,8 +117,9 @@ int plan_shot(sam2d_ctx *ctx, const sam2d_shot &sh, DeviceDomain *dom, ShotPlan 
 int make_plans(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots, const int *windows,
                std::vector<ShotPlan> &plans);
 int pick_variant(sam2d_ctx *ctx);
-int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *ntiles_out);
-int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int waves, int n0, int n1, int max_ng);
+int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *ntiles_out,
+                int nslots = 2);
+int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int n0, int n1, int max_ng);
 void tile_shape(int nnz, int nnx, int waves, int chunk, int &ngroups, int &nchunks);
 }  // namespace sam2d
 

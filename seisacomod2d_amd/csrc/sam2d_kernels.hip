@@ -22,72 +22,22 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdio.h>
 #include "sam2d_device.h"
+#include "sam2d_kcommon.h"
 
 namespace sam2d {
-
-#define SAM2D_A0 1.125f
-#define SAM2D_A1 (-1.0f / 24.0f)
-
-// Coefficients of the backward-staggered first difference D- at index i of
-// an n-point axis (fdoperator420/410, cpu_modelling_kernels.cpp:201-258):
-// 4th order on [2, n-2], 2nd order at 1 and n-1, zero at 0 and outside.
-__device__ __forceinline__ void dminus_coef(int i, int n, float &c0, float &c1)
-{
-    if (i >= 2 && i <= n - 2) { c0 = SAM2D_A0; c1 = SAM2D_A1; }
-    else if (i == 1 || i == n - 1) { c0 = 1.0f; c1 = 0.0f; }
-    else { c0 = 0.0f; c1 = 0.0f; }
-}
-// Forward-staggered D+ (fdoperator421/411, :260-316): 4th order on [1, n-3],
-// 2nd order at 0 and n-2, zero at n-1 and outside.
-__device__ __forceinline__ void dplus_coef(int i, int n, float &c0, float &c1)
-{
-    if (i >= 1 && i <= n - 3) { c0 = SAM2D_A0; c1 = SAM2D_A1; }
-    else if (i == 0 || i == n - 2) { c0 = 1.0f; c1 = 0.0f; }
-    else { c0 = 0.0f; c1 = 0.0f; }
-}
-
-// Descriptor pointers are generic; casting them into the global address space
-// makes the compiler emit global_* instead of flat_* memory instructions.
-// Flat accesses also count against lgkmcnt, so every lane-shuffle wait would
-// otherwise drain all loads in flight and serialise the column prefetch.
-#define SAM2D_GLOBAL __attribute__((address_space(1)))
-template <class T>
-__device__ __forceinline__ SAM2D_GLOBAL T *gp(T *p) { return (SAM2D_GLOBAL T *)p; }
-
-__device__ __forceinline__ float shfl_up(float v, int d) { return __shfl_up(v, d, 64); }
-__device__ __forceinline__ float shfl_dn(float v, int d) { return __shfl_down(v, d, 64); }
-
-// Virtual block id -> tile id.  Blocks are dispatched round-robin over the 8
-// XCDs (block b lands on XCD b % 8); giving each XCD a contiguous range of
-// tiles keeps z-neighbouring tiles, which read each other's halo rows, in
-// the same L2.
-__device__ __forceinline__ int xcd_tile(int v, int per_xcd)
-{
-    return (v & 7) * per_xcd + (v >> 3);
-}
-
-// Shot owning a flat tile id (shots are few; descriptors are uniform).
-__device__ __forceinline__ int shot_of_tile(const ShotDev *sh, int nshots, int tile)
-{
-    int lo = 0, hi = nshots - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (sh[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
 
 // Receiver gather of p = px + pz from the given state buffers
 // (cpu_record, cpu_modelling_kernels.cpp:24-31, trace-major output
 // as after cpu_transpose, :370-381).
-__device__ __forceinline__ void gather_receivers(const ShotDev *sh, int nshots, int par,
+__device__ __forceinline__ void gather_receivers(const ShotDev *sh, int nshots, int in_slot,
                                                  int slot, int gtid, int gstride)
 {
     for (int s = 0; s < nshots; ++s) {
         const ShotDev &d = sh[s];
-        const SAM2D_GLOBAL float *px = gp(d.st[par][0]);
-        const SAM2D_GLOBAL float *pz = gp(d.st[par][1]);
+        const SAM2D_GLOBAL float *px = gp(slot_field(d, in_slot, 0));
+        const SAM2D_GLOBAL float *pz = gp(slot_field(d, in_slot, 1));
         const SAM2D_GLOBAL int *gc = gp(d.gcell);
         SAM2D_GLOBAL float *seis = gp(d.seis);
         for (int g = gtid; g < d.ng; g += gstride) {
@@ -113,9 +63,9 @@ __device__ __forceinline__ void st(SAM2D_GLOBAL float *p, unsigned i, float v)
     else p[i] = v;
 }
 
-// One time step.  par selects the input buffer set (output is 1 - par).
-// rec_slot >= 0 asks for the receiver gather of the state held in the input
-// buffers (the previous step's result) into sample rec_slot.
+// One time step: state slot in_slot -> out_slot.  rec_slot >= 0 asks for the
+// receiver gather of the input state (the previous step's result) into
+// sample rec_slot.
 //
 // WAVES: z-strips per workgroup.  PF: columns of operands in flight (2 or 4).
 // NTS: non-temporal hint on the state stores.
@@ -132,14 +82,14 @@ __device__ __forceinline__ void st(SAM2D_GLOBAL float *p, unsigned i, float v)
 template <int WAVES, int PF, bool NTS>
 __global__ __launch_bounds__(WAVES * 64)
 void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
-            int ntiles, int step, int par, int rec_slot)
+            int ntiles, int step, int in_slot, int out_slot, int rec_slot)
 {
     static_assert(PF == 1 || PF == 2 || PF == 4, "PF must divide the unroll of 4");
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
 
     if (rec_slot >= 0)
-        gather_receivers(shots, nshots, par, rec_slot,
+        gather_receivers(shots, nshots, in_slot, rec_slot,
                          blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 
     for (int v = blockIdx.x; v < vtotal; v += gridDim.x) {
@@ -160,14 +110,14 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         const bool rown = (lane >= HALO_T) && (lane < HALO_T + OWN_Z) && (r < nnz);
         const unsigned rc = (unsigned)min(max(r, 0), nnz - 1);
 
-        const SAM2D_GLOBAL float *__restrict__ pxi = gp(d.st[par][0]);
-        const SAM2D_GLOBAL float *__restrict__ pzi = gp(d.st[par][1]);
-        const SAM2D_GLOBAL float *__restrict__ vxi = gp(d.st[par][2]);
-        const SAM2D_GLOBAL float *__restrict__ vzi = gp(d.st[par][3]);
-        SAM2D_GLOBAL float *__restrict__ pxo = gp(d.st[par ^ 1][0]);
-        SAM2D_GLOBAL float *__restrict__ pzo = gp(d.st[par ^ 1][1]);
-        SAM2D_GLOBAL float *__restrict__ vxo = gp(d.st[par ^ 1][2]);
-        SAM2D_GLOBAL float *__restrict__ vzo = gp(d.st[par ^ 1][3]);
+        const SAM2D_GLOBAL float *__restrict__ pxi = gp(slot_field(d, in_slot, 0));
+        const SAM2D_GLOBAL float *__restrict__ pzi = gp(slot_field(d, in_slot, 1));
+        const SAM2D_GLOBAL float *__restrict__ vxi = gp(slot_field(d, in_slot, 2));
+        const SAM2D_GLOBAL float *__restrict__ vzi = gp(slot_field(d, in_slot, 3));
+        SAM2D_GLOBAL float *__restrict__ pxo = gp(slot_field(d, out_slot, 0));
+        SAM2D_GLOBAL float *__restrict__ pzo = gp(slot_field(d, out_slot, 1));
+        SAM2D_GLOBAL float *__restrict__ vxo = gp(slot_field(d, out_slot, 2));
+        SAM2D_GLOBAL float *__restrict__ vzo = gp(slot_field(d, out_slot, 3));
         const SAM2D_GLOBAL float *__restrict__ KX = gp(d.KX);
         const SAM2D_GLOBAL float *__restrict__ KZ = gp(d.KZ);
         const SAM2D_GLOBAL float *__restrict__ BX = gp(d.BX);
@@ -269,9 +219,9 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
 }
 
 // Receiver gather alone (the last step's samples).
-__global__ void k_record(const ShotDev *__restrict__ shots, int nshots, int par, int slot)
+__global__ void k_record(const ShotDev *__restrict__ shots, int nshots, int in_slot, int slot)
 {
-    gather_receivers(shots, nshots, par, slot, blockIdx.x * blockDim.x + threadIdx.x,
+    gather_receivers(shots, nshots, in_slot, slot, blockIdx.x * blockDim.x + threadIdx.x,
                      gridDim.x * blockDim.x);
 }
 
@@ -318,23 +268,24 @@ void variant_name(int v, char *buf, int n)
 
 template <int W, int PF, bool NTS>
 static void launch_t(dim3 g, hipStream_t s, const ShotDev *d, int ns, int vt, int per, int nt,
-                     int step, int par, int slot)
+                     int step, int in, int out, int slot)
 {
-    hipLaunchKernelGGL((k_step<W, PF, NTS>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step, par, slot);
+    hipLaunchKernelGGL((k_step<W, PF, NTS>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step, in,
+                       out, slot);
 }
 
 template <int W>
 static void launch_w(int pf, int nts, dim3 g, hipStream_t s, const ShotDev *d, int ns, int vt,
-                     int per, int nt, int step, int par, int slot)
+                     int per, int nt, int step, int in, int out, int slot)
 {
-    if (pf == 2 && !nts) launch_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, par, slot);
-    else if (pf == 2) launch_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, par, slot);
-    else if (!nts) launch_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, par, slot);
-    else launch_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, par, slot);
+    if (pf == 2 && !nts) launch_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, in, out, slot);
+    else if (pf == 2) launch_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, in, out, slot);
+    else if (!nts) launch_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, in, out, slot);
+    else launch_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, in, out, slot);
 }
 
 hipError_t launch_step(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
-                       int step, int par, int rec_slot, hipStream_t stream)
+                       int step, int in_slot, int out_slot, int rec_slot, hipStream_t stream)
 {
     int waves, chunk, pf, nts;
     decode(variant, waves, chunk, pf, nts);
@@ -344,21 +295,21 @@ hipError_t launch_step(int variant, const ShotDev *d_shots, int nshots, int ntil
     if (grid_cap > 0 && grid > grid_cap) grid = (grid_cap + 7) / 8 * 8;
     const dim3 g(grid);
     switch (waves) {
-    case 1: launch_w<1>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
-    case 2: launch_w<2>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
-    case 4: launch_w<4>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
-    case 8: launch_w<8>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, par, rec_slot); break;
+    case 1: launch_w<1>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
+    case 2: launch_w<2>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
+    case 4: launch_w<4>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
+    case 8: launch_w<8>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_record(const ShotDev *d_shots, int nshots, int max_ng, int par, int slot,
+hipError_t launch_record(const ShotDev *d_shots, int nshots, int max_ng, int in_slot, int slot,
                          hipStream_t stream)
 {
     int blocks = (max_ng + 255) / 256;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_record, dim3(blocks), dim3(256), 0, stream, d_shots, nshots, par, slot);
+    hipLaunchKernelGGL(k_record, dim3(blocks), dim3(256), 0, stream, d_shots, nshots, in_slot, slot);
     return hipGetLastError();
 }
 

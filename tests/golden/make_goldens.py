@@ -14,6 +14,7 @@ output as tests/golden/<case>.npz:
            (its test_geometry.txt dump of shot 0, create_geometry.cpp:446-459)
   seis     (ng, real_nt) float32 pressure seismogram (the reference's
            <job>_seismogram.bin), possibly decimated in traces: trace_idx
+  geometry_text  the shot-file line or explicit-geometry lines of the card
 
 The reference fixes npml = 32 (cpumodelling.cpp:15), so every golden is at
 npml = 32.  Runs are single-shot: multi-shot runs of the reference overwrite
@@ -76,6 +77,7 @@ def run_reference(nx, nz, dx, dz, T, fd_dt, real_dt, fm, vp, rho, shot_line=None
     src = geo[0, ::-1].astype(np.float32)[None, :]
     rec = geo[1:, ::-1].astype(np.float32)
     ng = rec.shape[0]
+    run_reference.geometry_text = shot_line if shot_line else "\n".join(geom_lines)
     return src, rec, seis.reshape(ng, -1), log
 
 
@@ -116,6 +118,7 @@ def salt(nx, nz, h, seed):
 
 
 def save(name, **kw):
+    kw["geometry_text"] = np.array(run_reference.geometry_text)
     path = os.path.join(HERE, name + ".npz")
     np.savez_compressed(path, **kw)
     print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")

@@ -1,0 +1,51 @@
+"""GPU end-to-end test of the job-card driver SeisAcoMod2D_MI355X.exe: the
+reference's card, shot file and model files in, the reference's seismogram
+file out (<out>/<job>_seismogram.bin, trace-major float32), compared with the
+compiled reference's golden output (relative L2 per trace <= 1e-5)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, assert_seis_close, load_golden
+from test_host_cli import _write_card
+
+pytestmark = pytest.mark.gpu
+EXE = os.path.join(ROOT, "seisacomod2d_amd", "SeisAcoMod2D_MI355X.exe")
+
+
+@pytest.mark.parametrize("case", ["tiny_twolayer", "hetero_explicit", "rect_dtfactor"])
+def test_cli_reproduces_reference_output_file(tmp_path, case):
+    g = load_golden(case)
+    text = str(np.load(os.path.join(ROOT, "tests", "golden", case + ".npz"))["geometry_text"])
+    flag = 0 if text.count("\n") else 1
+    card = _write_card(str(tmp_path), g, flag, text)
+    r = subprocess.run([EXE, card], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    out = np.fromfile(os.path.join(str(tmp_path), "card_seismogram.bin"), np.float32)
+    out = out.reshape(g["rec_xz"].shape[0], -1)[g["trace_idx"]]
+    assert_seis_close(out, g["seis"], 1e-5, case)
+    # the restart file the reference only reads is written: one shot done
+    assert open(os.path.join(str(tmp_path), "card_checkpoint_rank0.txt")).read().strip() == "1"
+
+
+def test_cli_restart_and_abort_codes(tmp_path):
+    g = load_golden("tiny_twolayer")
+    d = str(tmp_path)
+    card = _write_card(d, g, 1, "300.000000,20.000000,50,0,10.000000,1\n600.000000,20.000000,50,0,10.000000,1",
+                       jobtype="Restart")
+    # Restart without the file -> the reference's Abort(-10)
+    r = subprocess.run([EXE, card], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 10
+    open(os.path.join(d, "card_checkpoint_rank0.txt"), "w").write("1\n")
+    r = subprocess.run([EXE, card, "--per-shot-files"], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert not os.path.exists(os.path.join(d, "card_shot0_seismogram.bin"))   # skipped by the restart
+    assert os.path.exists(os.path.join(d, "card_shot1_seismogram.bin"))
+    # CFL violation -> Abort(-5)
+    txt = open(card).read().replace('"FD DT" : "0.001"', '"FD DT" : "0.004"').replace(
+        '"DATA DT" : "0.001"', '"DATA DT" : "0.004"')
+    open(card, "w").write(txt)
+    r = subprocess.run([EXE, card], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 5
