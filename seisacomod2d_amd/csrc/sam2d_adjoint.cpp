@@ -117,7 +117,8 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     AdjointState &A = *ctx->adj;
     const TimeAxis &ta = ctx->ta;
     const int ns = (int)plans.size();
-    const int variant = pick_variant(ctx);
+    // Every forward state is needed in the ring: single-step kernels only.
+    const int variant = variant_single_step(pick_variant(ctx));
     int ntiles = 0;
     SAM2D_TRY(stage_batch(ctx, plans, variant, &ntiles, k + 1));
     if ((int)A.bufs.size() < ns) A.bufs.resize(ns);

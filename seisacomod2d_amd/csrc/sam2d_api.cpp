@@ -75,8 +75,27 @@ ShotBuffers::~ShotBuffers() { release(); }
 void ShotBuffers::release()
 {
     dev_free(state); dev_free(amp); dev_free(gcell); dev_free(seis);
+    dev_free(rcolptr); dev_free(rrow); dev_free(rid);
     state = nullptr; amp = nullptr; gcell = nullptr; seis = nullptr;
-    cells = 0; cap_state = 0; cap_nts = 0; cap_ng = 0; cap_seis = 0;
+    rcolptr = nullptr; rrow = nullptr; rid = nullptr;
+    cells = 0; cap_state = 0; cap_nts = 0; cap_ng = 0; cap_seis = 0; cap_cols = 0; cap_csr = 0;
+}
+
+int ShotBuffers::reserve_csr(int ncols, int ng)
+{
+    if (ncols + 1 > cap_cols) {
+        dev_free(rcolptr); rcolptr = nullptr; cap_cols = 0;
+        SAM2D_TRY(dev_alloc((void **)&rcolptr, sizeof(int) * (ncols + 1)));
+        cap_cols = ncols + 1;
+    }
+    const int need = ng > 0 ? ng : 1;
+    if (need > cap_csr) {
+        dev_free(rrow); dev_free(rid); rrow = nullptr; rid = nullptr; cap_csr = 0;
+        SAM2D_TRY(dev_alloc((void **)&rrow, sizeof(int) * need));
+        SAM2D_TRY(dev_alloc((void **)&rid, sizeof(int) * need));
+        cap_csr = need;
+    }
+    return SAM2D_OK;
 }
 
 int ShotBuffers::reserve(size_t ncells, int nslots, int nts, int ng, size_t nseis)
@@ -173,9 +192,9 @@ int plan_shot(sam2d_ctx *ctx, const sam2d_shot &sh, DeviceDomain *dom, ShotPlan 
 }
 
 // Tile geometry of one shot for a kernel variant.
-void tile_shape(int nnz, int nnx, int waves, int chunk, int &ngroups, int &nchunks)
+void tile_shape(int nnz, int nnx, int waves, int chunk, int own_rows, int &ngroups, int &nchunks)
 {
-    const int nstrips = (nnz + OWN_Z - 1) / OWN_Z;
+    const int nstrips = (nnz + own_rows - 1) / own_rows;
     ngroups = (nstrips + waves - 1) / waves;
     nchunks = (nnx + chunk - 1) / chunk;
 }
@@ -208,12 +227,28 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
         SAM2D_TRY(b.reserve(N, nslots, nts, p.ng, (size_t)p.ng * real_nt));
         SAM2D_TRY(upload(b.amp, p.amp.data(), sizeof(float) * nts, ctx->stream));
         SAM2D_TRY(upload(b.gcell, p.gcell.data(), sizeof(int) * p.ng, ctx->stream));
+        // Receivers by column (CSR) for the in-place recording of the
+        // two-step kernel.
+        std::vector<int> colptr(nnx + 1, 0), rrow(std::max(1, p.ng)), rid(std::max(1, p.ng));
+        for (int g = 0; g < p.ng; g++) colptr[p.gcell[g] / nnz + 1]++;
+        for (int c = 0; c < nnx; c++) colptr[c + 1] += colptr[c];
+        std::vector<int> fill(colptr.begin(), colptr.end() - 1);
+        for (int g = 0; g < p.ng; g++) {
+            const int c = p.gcell[g] / nnz;
+            const int e = fill[c]++;
+            rrow[e] = p.gcell[g] % nnz;
+            rid[e] = g;
+        }
+        SAM2D_TRY(b.reserve_csr(nnx, p.ng));
+        SAM2D_TRY(upload(b.rcolptr, colptr.data(), sizeof(int) * (nnx + 1), ctx->stream));
+        SAM2D_TRY(upload(b.rrow, rrow.data(), sizeof(int) * p.ng, ctx->stream));
+        SAM2D_TRY(upload(b.rid, rid.data(), sizeof(int) * p.ng, ctx->stream));
         HIP_TRY(launch_fill(b.slot(0), 4 * N, 0.0f, ctx->stream));
         if (p.ng) HIP_TRY(launch_fill(b.seis, (size_t)p.ng * real_nt, 0.0f, ctx->stream));
         ShotDev &d = desc[s];
         std::memset(&d, 0, sizeof d);
         d.nnz = nnz; d.nnx = nnx;
-        tile_shape(nnz, nnx, waves, chunk, d.ngroups, d.nchunks);
+        tile_shape(nnz, nnx, waves, chunk, variant_own_rows(variant), d.ngroups, d.nchunks);
         d.chunk = chunk;
         d.tile_begin = tiles;
         tiles += d.ngroups * d.nchunks;
@@ -225,6 +260,7 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
         d.state = b.state;
         d.ncell = N;
         d.gcell = b.gcell; d.seis = b.seis;
+        d.rcolptr = b.rcolptr; d.rrow = b.rrow; d.rid = b.rid;
         p.seis_dev = b.seis;
     }
     if (ns > ctx->cap_desc) {
@@ -251,17 +287,54 @@ static int rec_slot_for_kernel(const TimeAxis &ta, int n)
     return slot < ta.real_nt ? slot : -1;
 }
 
-// Enqueue steps [n0, n1) of the time loop (the tail record when n1 == fd_nt).
-int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int n0, int n1, int max_ng)
+// Receiver sample recorded after step n, or -1.
+static int rec_after(const TimeAxis &ta, int n)
 {
-    for (int n = n0; n < n1; n++)
-        HIP_TRY(launch_step(variant, ctx->d_desc, nshots, ntiles, ctx->grid_cap, n, n & 1,
-                            (n & 1) ^ 1, rec_slot_for_kernel(ctx->ta, n), ctx->stream));
-    if (n1 == ctx->ta.fd_nt) {
+    if ((n + 1) % ta.dt_factor != 0) return -1;
+    const int k = (n + 1) / ta.dt_factor - 1;
+    return k < ta.real_nt ? k : -1;
+}
+
+// Enqueue steps [n0, n1) of the time loop starting from state slot
+// start_slot; *end_slot receives the slot holding the final state.  When n1 is
+// the last step, the receivers of the last step are recorded too.
+// Single-step variants record step n-1 inside kernel n (deferred gather);
+// two-step variants record both of their steps in place, and an odd last step
+// runs the single-step kernel followed by the tail record.
+int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int n0, int n1, int max_ng,
+                  int start_slot, int *end_slot)
+{
+    int cur = start_slot;
+    int n = n0;
+    bool last_recorded = false;
+    if (variant_steps(variant) == 2) {
+        const int single = variant_single_step(variant);
+        for (; n + 1 < n1; n += 2) {
+            HIP_TRY(launch_step2(variant, ctx->d_desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
+                                 rec_after(ctx->ta, n), rec_after(ctx->ta, n + 1), ctx->stream));
+            cur ^= 1;
+            last_recorded = true;
+        }
+        if (n < n1) {
+            HIP_TRY(launch_step(single, ctx->d_desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
+                                -1, ctx->stream));
+            cur ^= 1;
+            n++;
+            last_recorded = false;
+        }
+    } else {
+        for (; n < n1; n++) {
+            HIP_TRY(launch_step(variant, ctx->d_desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
+                                rec_slot_for_kernel(ctx->ta, n), ctx->stream));
+            cur ^= 1;
+        }
+    }
+    if (n1 == ctx->ta.fd_nt && !last_recorded) {
         const int slot = rec_slot_for_kernel(ctx->ta, n1);
         if (slot >= 0 && max_ng > 0)
-            HIP_TRY(launch_record(ctx->d_desc, nshots, max_ng, n1 & 1, slot, ctx->stream));
+            HIP_TRY(launch_record(ctx->d_desc, nshots, max_ng, cur, slot, ctx->stream));
     }
+    if (end_slot) *end_slot = cur;
     return SAM2D_OK;
 }
 
@@ -282,7 +355,7 @@ int run_time_loop(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int max_n
         if (!exec) {
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_steps(ctx, nshots, ntiles, variant, 0, fd_nt, max_ng);
+            int rc = enqueue_steps(ctx, nshots, ntiles, variant, 0, fd_nt, max_ng, 0, nullptr);
             hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
             if (rc != SAM2D_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
             HIP_TRY(e);
@@ -301,7 +374,7 @@ int run_time_loop(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int max_n
         HIP_TRY(hipGraphLaunch(exec, ctx->stream));
         used_graph = true;
     } else {
-        SAM2D_TRY(enqueue_steps(ctx, nshots, ntiles, variant, 0, fd_nt, max_ng));
+        SAM2D_TRY(enqueue_steps(ctx, nshots, ntiles, variant, 0, fd_nt, max_ng, 0, nullptr));
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev1));
@@ -353,15 +426,13 @@ static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, flo
     const int real_nt = ctx->ta.real_nt;
     size_t off = 0;
     const int variant = pick_variant(ctx);
-    int waves, chunk;
-    variant_shape(variant, &waves, &chunk);
+    int fin = 0;    // slot holding the previous shot's final state
     for (size_t s = 0; s < plans.size(); s++) {
         std::vector<ShotPlan> one(1, plans[s]);
         // Keep the previous state: stage into slot 0 without its clearing.
         ShotBuffers *b0 = ctx->bufs.empty() || !ctx->bufs[0] ? nullptr : ctx->bufs[0].get();
         const size_t N = (size_t)one[0].dom->host.nnx * one[0].dom->host.nnz;
         std::vector<float> keep;
-        const int fin = ctx->ta.fd_nt & 1;
         if (s > 0 && b0) {
             // Bring the final state into set 0, then clear the first nnz cells.
             keep.resize(4 * N);
@@ -379,7 +450,7 @@ static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, flo
         }
         ctx->stats.cell_updates += (double)N * ctx->ta.fd_nt;
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        SAM2D_TRY(enqueue_steps(ctx, 1, ntiles, variant, 0, ctx->ta.fd_nt, one[0].ng));
+        SAM2D_TRY(enqueue_steps(ctx, 1, ntiles, variant, 0, ctx->ta.fd_nt, one[0].ng, 0, &fin));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(hipEventSynchronize(ctx->ev1));
         float ms = 0;
@@ -619,9 +690,9 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
         variant_shape(v, &waves, &chunk);
         int ntiles = 0;
         SAM2D_TRY(stage_batch(ctx, plans, v, &ntiles));
-        SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, v, 0, 2, 0));   // warm
+        SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, v, 0, 2, 0, 0, nullptr));   // warm
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, v, 2, 2 + steps, 0));
+        SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, v, 2, 2 + steps, 0, 0, nullptr));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(hipEventSynchronize(ctx->ev1));
         float ms = 0;

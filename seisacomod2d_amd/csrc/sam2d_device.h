@@ -9,6 +9,10 @@ namespace sam2d {
 constexpr int OWN_Z = 58;
 constexpr int HALO_T = 3;
 constexpr int HALO_B = 64 - OWN_Z - HALO_T;
+// The same for the kernel that advances two time steps per pass: the second
+// step's dependency cone needs six halo rows on each side.
+constexpr int OWN_Z2 = 52;
+constexpr int HALO_T2 = 6;
 
 // One resident shot: its padded domain, coefficient fields, ping-pong state
 // and receiver list, all device pointers.  Kernels read it from device memory
@@ -32,6 +36,9 @@ struct ShotDev {
     size_t ncell;          //   field f of slot s at state + (4*s + f)*ncell
     const int *gcell;      // [ng] receiver cells (ix*nnz + iz)
     float *seis;           // [ng][real_nt]
+    const int *rcolptr;    // [nnx+1] receivers of each column (CSR, by column)
+    const int *rrow;       // [ng] padded row of each entry
+    const int *rid;        // [ng] receiver index of each entry
 };
 
 __host__ __device__ inline float *slot_field(const ShotDev &d, int slot, int f)
@@ -70,6 +77,18 @@ hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int
 int num_variants();
 void variant_shape(int v, int *waves, int *chunk);
 void variant_name(int v, char *buf, int n);
+// Time steps one launch of variant v advances (1 or 2), and the rows a
+// wavefront owns for it.
+int variant_steps(int v);
+int variant_own_rows(int v);
+// The single-step variant with the same workgroup and chunk shape.
+int variant_single_step(int v);
+
+// Two time steps (step, step+1) of every staged shot: in_slot -> out_slot;
+// rec1 / rec2 >= 0 record the pressure after each step into that sample.
+hipError_t launch_step2(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
+                        int step, int in_slot, int out_slot, int rec1, int rec2,
+                        hipStream_t stream);
 
 // One time step of every staged shot: reads state slot in_slot, writes
 // out_slot; rec_sample >= 0 gathers the receivers of the in_slot state.

@@ -241,16 +241,32 @@ static const int kChunks[] = {16, 32, 64, 128, 256};
 static const int kPF[] = {2, 4};
 static const int kNTS[] = {0, 1};
 static const int nW = 4, nC = 5, nP = 2, nN = 2;
+static const int kChunks2[] = {32, 64, 128, 256, 512};
+static const int nSingle = nW * nC * nP * nN;     // single-step variants first
+static const int nDouble = nW * 5;                 // then two-step variants
 
-int num_variants() { return nW * nC * nP * nN; }
+int num_variants() { return nSingle + nDouble; }
 
-// Variant index -> (waves, chunk, prefetch, nt-stores); chunk varies fastest.
-static void decode(int v, int &w, int &c, int &pf, int &nts)
+// Variant index -> (steps per launch, waves, chunk, prefetch, nt-stores).
+static void decode(int v, int &w, int &c, int &pf, int &nts, int &steps)
 {
+    if (v >= nSingle) {
+        v -= nSingle;
+        steps = 2; pf = 2; nts = 0;
+        c = kChunks2[v % 5];
+        w = kWaves[v / 5];
+        return;
+    }
+    steps = 1;
     c = kChunks[v % nC]; v /= nC;
     w = kWaves[v % nW]; v /= nW;
     pf = kPF[v % nP]; v /= nP;
     nts = kNTS[v % nN];
+}
+static void decode(int v, int &w, int &c, int &pf, int &nts)
+{
+    int steps;
+    decode(v, w, c, pf, nts, steps);
 }
 
 void variant_shape(int v, int *waves, int *chunk)
@@ -259,11 +275,32 @@ void variant_shape(int v, int *waves, int *chunk)
     decode(v, *waves, *chunk, pf, nts);
 }
 
-void variant_name(int v, char *buf, int n)
+int variant_steps(int v)
 {
+    int w, c, pf, nts, steps;
+    decode(v, w, c, pf, nts, steps);
+    return steps;
+}
+
+int variant_own_rows(int v) { return variant_steps(v) == 2 ? OWN_Z2 : OWN_Z; }
+
+int variant_single_step(int v)
+{
+    if (v < nSingle) return v;
     int w, c, pf, nts;
     decode(v, w, c, pf, nts);
-    snprintf(buf, n, "waves%d_chunk%d_pf%d%s", w, c, pf, nts ? "_nts" : "");
+    int wi = 0, ci = 1;
+    for (int i = 0; i < nW; i++) if (kWaves[i] == w) wi = i;
+    for (int i = 0; i < nC; i++) if (kChunks[i] == c) ci = i;
+    return wi * nC + ci;   // prefetch 2, ordinary stores
+}
+
+void variant_name(int v, char *buf, int n)
+{
+    int w, c, pf, nts, steps;
+    decode(v, w, c, pf, nts, steps);
+    if (steps == 2) snprintf(buf, n, "twostep_waves%d_chunk%d", w, c);
+    else snprintf(buf, n, "waves%d_chunk%d_pf%d%s", w, c, pf, nts ? "_nts" : "");
 }
 
 template <int W, int PF, bool NTS>

@@ -186,3 +186,33 @@ def test_errors_are_statuses():
     assert_seis_close(s, g["seis"], TOL, "after errors")
     with pytest.raises(sam.SAM2DError):
         sam.Modeller(_grid(g, fdop=2), g["vp"], g["rho"])
+
+
+def _twostep_variants():
+    return [i for i, n in enumerate(sam.variants()) if n.startswith("twostep")]
+
+
+@pytest.mark.parametrize("case", ["c0_twolayer_npml32", "rect_dtfactor", "salt_small"])
+def test_twostep_kernel_bitwise_equals_single_step(case):
+    # Two time steps per pass (intermediate state kept in registers, receivers
+    # recorded in place) must reproduce the single-step kernel bit for bit.
+    g = load_golden(case)
+    with sam.Modeller(_grid(g), g["vp"], g["rho"]) as m:
+        m.set_option("variant", 11)                    # a single-step variant
+        ref = m.forward([_shot(g)])[0]
+        for v in _twostep_variants()[1::2]:
+            m.set_option("variant", v)
+            s = m.forward([_shot(g)])[0]
+            assert np.array_equal(s.view(np.int32), ref.view(np.int32)), sam.variants()[v]
+    assert_seis_close(ref[g["trace_idx"]], g["seis"], TOL, case)
+
+
+def test_twostep_batched_windows_bitwise():
+    g, shots, wins = _windows_case()
+    with sam.Modeller(_grid(g, npml=24), g["vp"], g["rho"]) as m:
+        m.set_option("variant", 11)
+        ref = m.forward_windows(shots, wins)
+        m.set_option("variant", _twostep_variants()[2])
+        out = m.forward_batched(shots, wins)
+    for a, b in zip(ref, out):
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
