@@ -1,20 +1,30 @@
 """Benchmark of the forward-modelling hot path on one MI355X.
 
-Workload (the headline configuration): plain FP32 forward on a two-layer model
-at 5000 x 5000 cells, dx = dz = 10 m (interface at z = 2500 m; vp 1500/3000 m/s,
-rho 1000/2000 kg/m^3), dt = 1 ms, 4000 steps, sponge 50 cells (padded grid
-5100 x 5100 = 26.01 M cells), 15 Hz Ricker, 64 shots evenly spaced along x at
-z = 20 m, 5000 surface receivers per shot, shots run one after another.
+Default workload (the headline configuration): plain FP32 forward on a
+two-layer model at 5000 x 5000 cells, dx = dz = 10 m (interface at z = 2500 m;
+vp 1500/3000 m/s, rho 1000/2000 kg/m^3), dt = 1 ms, 4000 steps, sponge 50 cells
+(padded grid 5100 x 5100 = 26.01 M cells), 15 Hz Ricker, 64 shots evenly
+spaced along x at z = 20 m, 5000 surface receivers per shot, shots run one
+after another.  One bench "step" is one shot: the full 4000-step time loop
+over the padded grid plus its 5000-trace gather.  The metric is padded
+cell-updates per second.  Inputs are resident in HBM before the timed region
+(the context uploads the model and builds the coefficient fields once);
+seismograms are written to a device buffer.
 
-One bench "step" is one shot: the full 4000-step time loop over the padded grid
-plus its 5000-trace gather.  The metric is padded cell-updates per second.
-Inputs are resident in HBM before the timed region (the context uploads the
-model and builds the coefficient fields once); seismograms are written to a
-device buffer.
+    python bench.py [--steps K] [--warmup W] [--no-cpu]
 
-    python bench.py [--steps K] [--warmup W] [--shots-per-job 64] [--no-cpu]
+Other workloads (each prints its own JSON line):
+    --workload batched    salt-type 3201 x 1201 model, 64 shots with aperture
+                          windows 800-2400 cells: shots/s of the batched launch
+                          against the per-shot loop on the same windows
+    --workload gradient   checkpointed adjoint gradient: 1201 x 601 smooth
+                          model with Gaussian anomalies, 16 shots, 3000 steps;
+                          and the salt model with 32 windowed shots, 8000
+                          steps, k = 50 / 100 / 200 under a 64 GiB budget
+                          (--quick: fewer shots): seconds per shot and the
+                          ratio to the forward
 
-Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+--gpus N with N > 1 exits: this build runs on one GPU.
 """
 import argparse
 import ctypes
@@ -37,6 +47,7 @@ FM = 15.0
 NSHOTS = 64
 BYTES_PER_CELL = 44          # px, pz, vx, vz read + written (32) + KX, BX, BZ read (12)
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E peak, GB/s
+METRIC = "Gcell-updates/s + %HBM peak; shots/s batched vs plain; gradient s/shot"
 
 
 def model():
@@ -92,31 +103,38 @@ def cpu_baseline(vp, rho, seconds_target=15.0):
                       f"5100x5100 padded grid, {th} OpenMP threads, setup excluded"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8, help="timed shots")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variant", type=int, default=-1)
-    ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=0,
-                    help="profiling runs only: shorten each shot to this many time steps")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="profiling calibration: one 1 GiB device-to-device copy, then exit")
-    ap.add_argument("--tuning", action="store_true", help="print per-variant autotune times")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on a single GPU; --gpus must be 1 "
-              "(shots are independent, multi-GPU scaling is a shot queue per device).",
-              file=sys.stderr)
-        sys.exit(2)
-    if args.copy_probe:
-        copy_probe()
-        return
+def measured_traffic(name="k_step2_config1_pmc.txt"):
+    """HBM bytes per launch of the dominant kernel from rocprofv3 --pmc
+    (FETCH_SIZE x2 + WRITE_SIZE x1, the copy-kernel calibration), as recorded in
+    profiles/<name> ("traffic_bytes_per_launch: <n>")."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
 
-    import seisacomod2d_amd as sam
 
+def copy_probe(nbytes=1 << 30):
+    """Calibration for rocprofv3's FETCH_SIZE / WRITE_SIZE: an elementwise
+    kernel (torch.neg) that reads nbytes and writes nbytes, run 3 times."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
+                      "repeats": 3, "kernel": "torch.neg elementwise"}))
+
+
+# --------------------------------------------------------------- config 1
+def bench_plain(args, sam):
     vp, rho = model()
     t_rec = T_REC if args.profile_steps <= 0 else args.profile_steps * DT
     grid = sam.Grid(NX, NZ, H, H, t_rec, DT, DT, FM, npml=NPML)
@@ -132,16 +150,14 @@ def main():
             print(f"# autotune picked {sam.variants()[best]}", file=sys.stderr, flush=True)
             if args.tuning:
                 for name, ms in zip(sam.variants(), m.tuning()):
-                    print(f"#   {name:24s} {ms * 1e3:8.1f} us/step", file=sys.stderr)
+                    print(f"#   {name:28s} {ms * 1e3:8.1f} us/step", file=sys.stderr)
         ng = all_shots[0].nrec
-        per_shot = ng * m.real_nt
-        seis = hip.malloc(per_shot * 4)
+        seis = hip.malloc(ng * m.real_nt * 4)
         cells = m.nnx * m.nnz
         for i in range(args.warmup):
             m.forward_dev([all_shots[i % NSHOTS]], seis)
         hip.sync()
-        k_ms = 0.0
-        launches = 0
+        k_ms, launches = 0.0, 0
         t0 = time.perf_counter()
         for i in range(args.steps):
             m.forward_dev([all_shots[(args.warmup + i) % NSHOTS]], seis)
@@ -151,16 +167,19 @@ def main():
             print(f"# shot {i + 1}/{args.steps} {st['wall_ms']:.1f} ms", file=sys.stderr, flush=True)
         hip.sync()
         wall = time.perf_counter() - t0
-        variant = sam.variants()[m.stats()["variant"]]
-        used_graph = m.stats()["used_graph"]
+        variant = sam.variants()[st["variant"]]
+        spl = st["steps_per_launch"]
+        used_graph = st["used_graph"]
         hip.free(seis)
 
     cell_updates = float(cells) * m.fd_nt * args.steps
     gcells = cell_updates / wall / 1e9
-    launch_ms = k_ms / max(launches, 1)
-    achieved = BYTES_PER_CELL * cells / (launch_ms * 1e-3) / 1e9   # GB/s per launch
+    launch_ms = k_ms / max(launches, 1)                     # one launch = spl time steps
+    bytes_per_launch = BYTES_PER_CELL * cells * spl          # algorithmic
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    kname = "k_step2" if spl == 2 else "k_step"
     out = {
-        "metric": "Gcell-updates/s (padded cells x time steps), plain FP32 forward",
+        "metric": METRIC,
         "value": round(gcells, 3),
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
@@ -174,50 +193,190 @@ def main():
         "data": "synthetic two-layer model (analytic), 64 shot positions cycled",
         "config": {"workload": "config1_plain_forward_5000x5000_nt4000_npml50",
                    "grid": [NX, NZ], "padded": [m.nnx, m.nnz], "nt": m.fd_nt, "receivers": ng,
-                   "shots_per_job": NSHOTS, "kernel_variant": variant, "hipgraph": used_graph},
+                   "shots_per_job": NSHOTS, "kernel_variant": variant, "hipgraph": used_graph,
+                   "profile_run": args.profile_steps > 0},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_step", "bytes_per_cell_update": BYTES_PER_CELL,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(f"{kname}_config1_pmc.txt"),
+                     "kernel": kname, "steps_per_launch": spl,
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(launch_ms * 1e3, 2)},
         "job_seconds_64_shots": round(wall / args.steps * NSHOTS, 2),
     }
-    traffic = _measured_traffic()
-    if traffic:
-        out["roofline"]["traffic"] = traffic
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(vp, rho)
     print(json.dumps(out), flush=True)
 
 
-def copy_probe(nbytes=1 << 30):
-    """A plain device-to-device copy of known size (a blit kernel), so that
-    rocprofv3's FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
-    hip = _Hip()
-    a, b = hip.malloc(nbytes), hip.malloc(nbytes)
-    for _ in range(3):
-        rc = hip.L.hipMemcpy(ctypes.c_void_p(b), ctypes.c_void_p(a), ctypes.c_size_t(nbytes), 3)
-        if rc != 0:
-            raise RuntimeError(f"hipMemcpy failed: {rc}")
-    hip.sync()
-    hip.free(a)
-    hip.free(b)
-    print(json.dumps({"copy_probe_bytes": nbytes, "repeats": 3}))
+# --------------------------------------------------------------- config 2
+def salt_model(nx=3201, nz=1201, h=7.62, seed=2026):
+    """Water to z = 1500 m, then 1800 + 0.5 (z - 1500) m/s, a seeded random
+    polygon salt body at 4500 m/s; Gardner density rho = 310 vp^0.25."""
+    rng = np.random.default_rng(seed)
+    z = (np.arange(nz) * h)[None, :]
+    x = (np.arange(nx) * h)[:, None]
+    vp = np.where(z < 1500.0, 1500.0, 1800.0 + 0.5 * (z - 1500.0)) + 0.0 * x
+    cx, cz = nx * h * rng.uniform(0.35, 0.65), nz * h * rng.uniform(0.55, 0.7)
+    R = 0.18 * nx * h
+    ang = np.arctan2(z - cz, x - cx)
+    rad = R * np.ones_like(ang)
+    for k in range(1, 6):
+        rad += R * rng.uniform(-0.12, 0.12) * np.cos(k * ang + rng.uniform(0, 2 * np.pi))
+    rad *= np.where(z < cz, 1.0, 0.6)            # flatter base
+    vp = np.where(np.hypot(x - cx, z - cz) < rad, 4500.0, vp)
+    vp = vp.astype(np.float32)
+    rho = (310.0 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+    return vp, rho
 
 
-def _measured_traffic():
-    """HBM bytes per k_step launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel calibration), as recorded in
-    profiles/k_step_config1_pmc.txt ("traffic_bytes_per_launch: <n>")."""
-    path = os.path.join(ROOT, "profiles", "k_step_config1_pmc.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            try:
-                return float(line.split(":", 1)[1])
-            except ValueError:
-                return None
-    return None
+def salt_shots(sam, n, nx=3201, h=7.62, seed=7, zr=15.24):
+    rng = np.random.default_rng(seed)
+    out, wins = [], []
+    for k in range(n):
+        w = int(rng.integers(800, 2401))
+        cx = int((k + 0.5) * nx / n)
+        wx0 = min(max(cx - w // 2, 0), nx - w)
+        rec = np.stack([(wx0 + np.arange(w)) * h + 0.5 * h, np.full(w, zr)], 1).astype(np.float32)
+        out.append(sam.Shot((cx + 0.5) * h, zr, rec))
+        wins.append((wx0, w))
+    return out, wins
+
+
+def bench_batched(args, sam):
+    nshots = 16 if args.quick else 64
+    vp, rho = salt_model()
+    grid = sam.Grid(3201, 1201, 7.62, 7.62, 4.0 if not args.quick else 1.0, 0.0005, 0.0005, 15.0, npml=60)
+    sh, wins = salt_shots(sam, nshots)
+    cells = sum((w + 120) * (1201 + 120) for _, w in wins)
+    with sam.Modeller(grid, vp, rho) as m:
+        # per-shot loop on the windows, variant tuned on one window
+        m.autotune(windows=wins[:1], nshots=1, steps=6)
+        m.forward_windows(sh[:1], wins[:1])                     # warm
+        t0 = time.perf_counter()
+        m.forward_windows(sh, wins)
+        t_plain = time.perf_counter() - t0
+        vplain = sam.variants()[m.stats()["variant"]]
+        print(f"# per-shot loop: {t_plain:.2f} s ({vplain})", file=sys.stderr, flush=True)
+        # batched: variant tuned on the resident batch, for a few batch sizes
+        t_batch, best = None, None
+        for mb in (args.max_batch,) if args.max_batch else (0, 8, 16):
+            m.set_option("max_batch", mb)
+            m.autotune(windows=wins, nshots=nshots, steps=6)
+            m.forward_batched(sh, wins)            # warm: graphs captured and cached
+            t0 = time.perf_counter()
+            m.forward_batched(sh, wins)
+            t = time.perf_counter() - t0
+            v = sam.variants()[m.stats()["variant"]]
+            print(f"# batched max_batch={mb}: {t:.2f} s ({v})", file=sys.stderr, flush=True)
+            if t_batch is None or t < t_batch:
+                t_batch, best = t, (mb, v)
+        nt = m.fd_nt
+    out = {
+        "metric": METRIC,
+        "value": round(nshots / t_batch, 3), "unit": "shots/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic salt-type model (seeded)",
+        "config": {"workload": f"config2_batched_3201x1201_nt{nt}_{nshots}shots_windows800-2400",
+                   "batched_max_batch": best[0], "batched_variant": best[1], "plain_variant": vplain},
+        "plain_shots_per_s": round(nshots / t_plain, 3),
+        "speedup_vs_plain": round(t_plain / t_batch, 3),
+        "batched_gcells": round(cells * nt / t_batch / 1e9, 2),
+        "plain_gcells": round(cells * nt / t_plain / 1e9, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+# --------------------------------------------------------------- configs 3, 4
+def anomaly_model(nx=1201, nz=601, h=10.0, seed=3):
+    """Background vp = 2000 + 0.6 z plus six seeded Gaussian anomalies
+    (+-300 m/s, sigma 150 m); rho = 2000."""
+    rng = np.random.default_rng(seed)
+    z = (np.arange(nz) * h)[None, :]
+    x = (np.arange(nx) * h)[:, None]
+    bg = (2000.0 + 0.6 * z + 0.0 * x)
+    true = bg.copy()
+    for k in range(6):
+        cx, cz = rng.uniform(0.15, 0.85) * nx * h, rng.uniform(0.25, 0.75) * nz * h
+        true += rng.choice([-300.0, 300.0]) * np.exp(-((x - cx) ** 2 + (z - cz) ** 2) / (2 * 150.0 ** 2))
+    rho = np.full((nx, nz), 2000.0, np.float32)
+    return true.astype(np.float32), bg.astype(np.float32), rho
+
+
+def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label):
+    res = []
+    with sam.Modeller(grid, vtrue, rho) as mt:
+        obs = mt.forward_windows(sh, wins) if wins else mt.forward(sh)
+    with sam.Modeller(grid, v0, rho) as m:
+        m.set_option("variant", sam.variants().index("waves4_chunk256_pf2"))
+        d = m.forward_windows(sh, wins) if wins else m.forward(sh)   # warm
+        t0 = time.perf_counter()
+        d = m.forward_windows(sh, wins) if wins else m.forward(sh)
+        t_fwd = time.perf_counter() - t0
+        adj = [a - b for a, b in zip(d, obs)]
+        for k in ks:
+            t0 = time.perf_counter()
+            gvp, grho = m.gradient(sh, adj, ckpt_k=k, windows=wins)
+            t_g = time.perf_counter() - t0
+            res.append({"workload": label, "ckpt_k": k, "shots": len(sh),
+                        "forward_s_per_shot": round(t_fwd / len(sh), 4),
+                        "gradient_s_per_shot": round(t_g / len(sh), 4),
+                        "gradient_s_per_batch": round(t_g, 3),
+                        "ratio_to_forward": round(t_g / t_fwd, 3),
+                        "misfit": float(sum(0.5 * np.sum(a.astype(np.float64) ** 2) for a in adj)),
+                        "grad_norm_vp": float(np.linalg.norm(gvp))})
+            print(f"# {label} k={k}: gradient {t_g:.2f} s, forward {t_fwd:.2f} s, "
+                  f"ratio {t_g / t_fwd:.2f}", file=sys.stderr, flush=True)
+    return res
+
+
+def bench_gradient(args, sam):
+    out = []
+    vtrue, v0, rho = anomaly_model()
+    n3 = 4 if args.quick else 16
+    grid = sam.Grid(1201, 601, 10.0, 10.0, 3.0, 0.001, 0.001, 10.0, npml=32)
+    rec = np.stack([np.arange(1201) * 10.0, np.full(1201, 10.0)], 1).astype(np.float32)
+    sh = [sam.Shot(10.0 * int((k + 0.5) * 1201 / n3), 10.0, rec) for k in range(n3)]
+    out += _grad_case(sam, grid, vtrue, v0, rho, sh, None, [100], "config3_gradient_1201x601_nt3000")
+    # config 4: salt model, smoothed starting model, windowed shots
+    vs, rs = salt_model()
+    from scipy.ndimage import gaussian_filter
+    v4 = gaussian_filter(vs.astype(np.float64), 20.0).astype(np.float32)
+    n4 = 4 if args.quick else 32
+    grid4 = sam.Grid(3201, 1201, 7.62, 7.62, 4.0, 0.0005, 0.0005, 15.0, npml=60)
+    sh4, w4 = salt_shots(sam, n4, seed=11)
+    out += _grad_case(sam, grid4, vs, v4, rs, sh4, w4, [50, 100, 200] if not args.quick else [200],
+                      "config4_gradient_3201x1201_nt8000_windows")
+    print(json.dumps({"metric": METRIC,
+                      "unit": "s/shot", "higher_is_better": False, "dtype": "f32",
+                      "results": out}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8, help="timed shots")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--workload", default="plain", choices=["plain", "batched", "gradient"])
+    ap.add_argument("--quick", action="store_true", help="batched/gradient: fewer shots")
+    ap.add_argument("--max-batch", type=int, default=0)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="profiling runs only: shorten each shot to this many time steps")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="profiling calibration: one 1 GiB device-to-device copy, then exit")
+    ap.add_argument("--tuning", action="store_true", help="print per-variant autotune times")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on a single GPU; --gpus must be 1 "
+              "(shots are independent, multi-GPU scaling is a shot queue per device).",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
+    import seisacomod2d_amd as sam
+    {"plain": bench_plain, "batched": bench_batched, "gradient": bench_gradient}[args.workload](args, sam)
 
 
 if __name__ == "__main__":

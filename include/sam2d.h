@@ -86,6 +86,7 @@ typedef struct sam2d_stats {
     long long launches;      /* step-kernel launches                         */
     int    used_graph;       /* 1 when the time loop ran as a hipGraph       */
     int    variant;          /* kernel variant index chosen (autotuner)      */
+    int    steps_per_launch; /* time steps one step-kernel launch advances   */
 } sam2d_stats;
 
 typedef struct sam2d_ctx sam2d_ctx;

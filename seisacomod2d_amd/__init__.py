@@ -54,7 +54,8 @@ class _Dims(ctypes.Structure):
 class _Stats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
                 ("cell_updates", ctypes.c_double), ("launches", ctypes.c_longlong),
-                ("used_graph", ctypes.c_int), ("variant", ctypes.c_int)]
+                ("used_graph", ctypes.c_int), ("variant", ctypes.c_int),
+                ("steps_per_launch", ctypes.c_int)]
 
 
 def lib():
@@ -207,7 +208,8 @@ class Modeller:
         s = _Stats()
         _check(lib().sam2d_get_stats(self._h, ctypes.byref(s)))
         return dict(kernel_ms=s.kernel_ms, wall_ms=s.wall_ms, cell_updates=s.cell_updates,
-                    launches=s.launches, used_graph=bool(s.used_graph), variant=s.variant)
+                    launches=s.launches, used_graph=bool(s.used_graph), variant=s.variant,
+                    steps_per_launch=s.steps_per_launch)
 
     def _out(self, shots):
         n = sum(s.nrec for s in shots) * self.real_nt

@@ -231,6 +231,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     ctx->stats.kernel_ms += ms;
     ctx->stats.launches += (long long)ta.fd_nt * 3;
     ctx->stats.variant = variant;
+    ctx->stats.steps_per_launch = 1;
 
     // 3. gradients to the host, chain rule
     for (int s = 0; s < ns; s++) {

@@ -381,7 +381,9 @@ int run_time_loop(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int max_n
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.kernel_ms += ms;
-    ctx->stats.launches += fd_nt;
+    const int spl = variant_steps(variant);
+    ctx->stats.launches += (fd_nt + spl - 1) / spl;
+    ctx->stats.steps_per_launch = spl;
     ctx->stats.used_graph = used_graph ? 1 : 0;
     ctx->stats.variant = variant;
     return SAM2D_OK;
@@ -456,7 +458,8 @@ static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, flo
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->stats.kernel_ms += ms;
-        ctx->stats.launches += ctx->ta.fd_nt;
+        ctx->stats.launches += (ctx->ta.fd_nt + variant_steps(variant) - 1) / variant_steps(variant);
+        ctx->stats.steps_per_launch = variant_steps(variant);
         const size_t n = (size_t)one[0].ng * real_nt;
         if (n) HIP_TRY(hipMemcpy(seis + off, one[0].seis_dev, n * 4, hipMemcpyDeviceToHost));
         off += n;
