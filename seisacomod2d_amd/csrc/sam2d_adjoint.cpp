@@ -38,7 +38,9 @@ struct AdjointState {
     std::vector<std::unique_ptr<AdjShotBuffers>> bufs;
     AdjDev *d_desc = nullptr;
     int cap_desc = 0;
-    ~AdjointState() { dev_free(d_desc); }
+    ShotDev *d_fwd = nullptr;      // forward-sweep descriptors (two-step tile shapes)
+    int cap_fwd = 0;
+    ~AdjointState() { dev_free(d_desc); dev_free(d_fwd); }
 };
 
 static int grow(void **p, size_t &cap, size_t need)
@@ -196,20 +198,47 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         return SAM2D_OK;
     };
 
-    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-    // 1. forward with checkpoints
-    const bool want_seis = seis_out != nullptr;
-    for (int n = 0; n < ta.fd_nt; n++) {
-        if (n % k == 0) SAM2D_TRY(save_or_restore(n / k, n & 1, true));
-        int rs = -1;
-        if (want_seis && n >= 1 && n % ta.dt_factor == 0 && n / ta.dt_factor - 1 < ta.real_nt)
-            rs = n / ta.dt_factor - 1;
-        HIP_TRY(launch_step(variant, ctx->d_desc, ns, ntiles, ctx->grid_cap, n, n & 1, (n & 1) ^ 1,
-                            rs, ctx->stream));
+    // Forward-sweep kernel: the context's variant when it advances T > 1
+    // steps per launch and k is a multiple of T (every checkpoint then falls
+    // on a launch boundary), on descriptors with that variant's tile shapes;
+    // otherwise the single-step variant of the recomputation.
+    int fwd_variant = pick_variant(ctx);
+    const ShotDev *fwd_desc = ctx->d_desc;
+    int fwd_tiles = ntiles;
+    if (variant_steps(fwd_variant) > 1 && k % variant_steps(fwd_variant) == 0) {
+        int w2, c2;
+        variant_shape(fwd_variant, &w2, &c2);
+        std::vector<ShotDev> d2(ctx->staged_desc.begin(), ctx->staged_desc.begin() + ns);
+        int t2 = 0;
+        for (auto &dd : d2) {
+            tile_shape(dd.nnz, dd.nnx, w2, c2, variant_own_rows(fwd_variant), dd.ngroups, dd.nchunks);
+            dd.chunk = c2;
+            dd.tile_begin = t2;
+            t2 += dd.ngroups * dd.nchunks;
+        }
+        if (ns > A.cap_fwd) {
+            dev_free(A.d_fwd); A.d_fwd = nullptr; A.cap_fwd = 0;
+            SAM2D_TRY(dev_alloc((void **)&A.d_fwd, sizeof(ShotDev) * ns));
+            A.cap_fwd = ns;
+        }
+        SAM2D_TRY(upload(A.d_fwd, d2.data(), sizeof(ShotDev) * ns, ctx->stream));
+        fwd_desc = A.d_fwd;
+        fwd_tiles = t2;
+    } else {
+        fwd_variant = variant;
     }
-    if (want_seis && ta.fd_nt % ta.dt_factor == 0 && ta.fd_nt / ta.dt_factor - 1 < ta.real_nt && max_ng)
-        HIP_TRY(launch_record(ctx->d_desc, ns, max_ng, ta.fd_nt & 1, ta.fd_nt / ta.dt_factor - 1,
-                              ctx->stream));
+
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    // 1. forward with checkpoints: checkpoint m holds the state entering
+    //    step m*k
+    const bool want_seis = seis_out != nullptr;
+    int cur = 0;
+    for (int m = 0; m < nseg; m++) {
+        SAM2D_TRY(save_or_restore(m, cur, true));
+        const int n0 = m * k, n1 = std::min(ta.fd_nt, n0 + k);
+        SAM2D_TRY(enqueue_steps_on(ctx, fwd_desc, ns, fwd_tiles, fwd_variant, n0, n1,
+                                   want_seis ? max_ng : 0, cur, &cur));
+    }
     // 2. backward by segments
     int lam = 0;
     for (int m = nseg - 1; m >= 0; m--) {
@@ -229,7 +258,8 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.kernel_ms += ms;
-    ctx->stats.launches += (long long)ta.fd_nt * 3;
+    ctx->stats.launches += (long long)ta.fd_nt * 2 +
+                           (ta.fd_nt + variant_steps(fwd_variant) - 1) / variant_steps(fwd_variant);
     ctx->stats.variant = variant;
     ctx->stats.steps_per_launch = 1;
 

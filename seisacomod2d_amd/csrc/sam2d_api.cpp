@@ -74,26 +74,18 @@ ShotBuffers::~ShotBuffers() { release(); }
 
 void ShotBuffers::release()
 {
-    dev_free(state); dev_free(amp); dev_free(gcell); dev_free(seis);
-    dev_free(rcolptr); dev_free(rrow); dev_free(rid);
-    state = nullptr; amp = nullptr; gcell = nullptr; seis = nullptr;
-    rcolptr = nullptr; rrow = nullptr; rid = nullptr;
-    cells = 0; cap_state = 0; cap_nts = 0; cap_ng = 0; cap_seis = 0; cap_cols = 0; cap_csr = 0;
+    dev_free(state); dev_free(amp); dev_free(gcell); dev_free(seis); dev_free(band);
+    state = nullptr; amp = nullptr; gcell = nullptr; seis = nullptr; band = nullptr;
+    cells = 0; cap_state = 0; cap_nts = 0; cap_ng = 0; cap_seis = 0; cap_band = 0;
 }
 
-int ShotBuffers::reserve_csr(int ncols, int ng)
+int ShotBuffers::reserve_band(size_t n)
 {
-    if (ncols + 1 > cap_cols) {
-        dev_free(rcolptr); rcolptr = nullptr; cap_cols = 0;
-        SAM2D_TRY(dev_alloc((void **)&rcolptr, sizeof(int) * (ncols + 1)));
-        cap_cols = ncols + 1;
-    }
-    const int need = ng > 0 ? ng : 1;
-    if (need > cap_csr) {
-        dev_free(rrow); dev_free(rid); rrow = nullptr; rid = nullptr; cap_csr = 0;
-        SAM2D_TRY(dev_alloc((void **)&rrow, sizeof(int) * need));
-        SAM2D_TRY(dev_alloc((void **)&rid, sizeof(int) * need));
-        cap_csr = need;
+    if (n == 0) n = 1;
+    if (n > cap_band) {
+        dev_free(band); band = nullptr; cap_band = 0;
+        SAM2D_TRY(dev_alloc((void **)&band, sizeof(float) * n));
+        cap_band = n;
     }
     return SAM2D_OK;
 }
@@ -206,6 +198,22 @@ int pick_variant(sam2d_ctx *ctx)
     return ctx->default_variant;
 }
 
+// Columns [lo, hi] around the middle where all four x-profiles
+// (prof_x[4][nnx]) are exactly 1; lo > hi when the middle column is not.
+static void ones_range(const std::vector<float> &prof_x, int nnx, int &lo, int &hi)
+{
+    auto ones = [&](int c) {
+        for (int k = 0; k < 4; k++)
+            if (prof_x[(size_t)k * nnx + c] != 1.0f) return false;
+        return true;
+    };
+    const int mid = nnx / 2;
+    if (!ones(mid)) { lo = 1; hi = 0; return; }
+    lo = hi = mid;
+    while (lo > 0 && ones(lo - 1)) lo--;
+    while (hi < nnx - 1 && ones(hi + 1)) hi++;
+}
+
 // Make `plans` resident: buffers, descriptors, zeroed state and seismograms.
 int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *ntiles_out,
                 int nslots)
@@ -222,27 +230,25 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
         DeviceDomain *dom = p.dom;
         const int nnz = dom->host.nnz, nnx = dom->host.nnx;
         const size_t N = (size_t)nnz * nnx;
+        if (variant_steps(variant) > 1 && N >= ((size_t)1 << 30))
+            return fail(SAM2D_ERR_PARAM, "grid of %zu cells too large for a multi-step variant "
+                        "(32-bit byte offsets); choose a single-step variant", N);
         if (!ctx->bufs[s]) ctx->bufs[s].reset(new ShotBuffers());
         ShotBuffers &b = *ctx->bufs[s];
         SAM2D_TRY(b.reserve(N, nslots, nts, p.ng, (size_t)p.ng * real_nt));
         SAM2D_TRY(upload(b.amp, p.amp.data(), sizeof(float) * nts, ctx->stream));
         SAM2D_TRY(upload(b.gcell, p.gcell.data(), sizeof(int) * p.ng, ctx->stream));
-        // Receivers by column (CSR) for the in-place recording of the
-        // two-step kernel.
-        std::vector<int> colptr(nnx + 1, 0), rrow(std::max(1, p.ng)), rid(std::max(1, p.ng));
-        for (int g = 0; g < p.ng; g++) colptr[p.gcell[g] / nnz + 1]++;
-        for (int c = 0; c < nnx; c++) colptr[c + 1] += colptr[c];
-        std::vector<int> fill(colptr.begin(), colptr.end() - 1);
+        // Rows spanned by the receivers: the multi-step kernels store the
+        // intermediate pressure of these rows into a band buffer (two
+        // parities x two slices of band_h x nnx) for the deferred gather.
+        int rlo = 1, rhi = 0;
         for (int g = 0; g < p.ng; g++) {
-            const int c = p.gcell[g] / nnz;
-            const int e = fill[c]++;
-            rrow[e] = p.gcell[g] % nnz;
-            rid[e] = g;
+            const int row = p.gcell[g] % nnz;
+            if (g == 0 || row < rlo) rlo = row;
+            if (g == 0 || row > rhi) rhi = row;
         }
-        SAM2D_TRY(b.reserve_csr(nnx, p.ng));
-        SAM2D_TRY(upload(b.rcolptr, colptr.data(), sizeof(int) * (nnx + 1), ctx->stream));
-        SAM2D_TRY(upload(b.rrow, rrow.data(), sizeof(int) * p.ng, ctx->stream));
-        SAM2D_TRY(upload(b.rid, rid.data(), sizeof(int) * p.ng, ctx->stream));
+        const int band_h = p.ng > 0 ? rhi - rlo + 1 : 0;
+        SAM2D_TRY(b.reserve_band((size_t)4 * band_h * nnx));
         HIP_TRY(launch_fill(b.slot(0), 4 * N, 0.0f, ctx->stream));
         if (p.ng) HIP_TRY(launch_fill(b.seis, (size_t)p.ng * real_nt, 0.0f, ctx->stream));
         ShotDev &d = desc[s];
@@ -254,13 +260,15 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
         tiles += d.ngroups * d.nchunks;
         d.src_x = p.src_x; d.src_z = p.src_z;
         d.nts = nts; d.ng = p.ng; d.real_nt = real_nt;
+        ones_range(dom->host.prof_x, nnx, d.xin_lo, d.xin_hi);
+        d.rrow_lo = rlo; d.rrow_hi = rhi; d.band_h = band_h;
         d.amp = b.amp;
         d.KX = dom->KX; d.KZ = dom->KZ; d.BX = dom->BX; d.BZ = dom->BZ;
         d.prof_x = dom->prof_x; d.prof_z = dom->prof_z;
         d.state = b.state;
         d.ncell = N;
         d.gcell = b.gcell; d.seis = b.seis;
-        d.rcolptr = b.rcolptr; d.rrow = b.rrow; d.rid = b.rid;
+        d.band = b.band;
         p.seis_dev = b.seis;
     }
     if (ns > ctx->cap_desc) {
@@ -296,43 +304,80 @@ static int rec_after(const TimeAxis &ta, int n)
 }
 
 // Enqueue steps [n0, n1) of the time loop starting from state slot
-// start_slot; *end_slot receives the slot holding the final state.  When n1 is
-// the last step, the receivers of the last step are recorded too.
-// Single-step variants record step n-1 inside kernel n (deferred gather);
-// two-step variants record both of their steps in place, and an odd last step
-// runs the single-step kernel followed by the tail record.
+// start_slot; *end_slot receives the slot holding the final state.
+// Single-step variants record step n-1 inside kernel n (deferred gather), and
+// the receivers of the last step when n1 is the last step.  Multi-step
+// variants (T = 2 or 3 steps per launch) gather the receivers of each launch
+// at the start of the next one (intermediate steps from the band buffer, the
+// last from the stored state); fewer than T remaining steps run the
+// single-step kernel; a final gather launch records what is still pending.
 int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int n0, int n1, int max_ng,
                   int start_slot, int *end_slot)
 {
+    return enqueue_steps_on(ctx, ctx->d_desc, nshots, ntiles, variant, n0, n1, max_ng, start_slot,
+                            end_slot);
+}
+
+int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int nshots, int ntiles, int variant,
+                     int n0, int n1, int max_ng, int start_slot, int *end_slot)
+{
+    const TimeAxis &ta = ctx->ta;
     int cur = start_slot;
     int n = n0;
-    bool last_recorded = false;
-    if (variant_steps(variant) == 2) {
+    const int T = variant_steps(variant);
+    if (T > 1) {
         const int single = variant_single_step(variant);
-        for (; n + 1 < n1; n += 2) {
-            HIP_TRY(launch_step2(variant, ctx->d_desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
-                                 rec_after(ctx->ta, n), rec_after(ctx->ta, n + 1), ctx->stream));
+        const bool kza = ta.dtdx == ta.dtdz;
+        // Samples still to gather: band slices 0 / 1 of parity band_w ^ 1,
+        // and the current state.
+        int pend0 = -1, pend1 = -1, pend_state = -1, band_w = 0;
+        for (; n + T - 1 < n1; n += T) {
+            const int r0 = rec_after(ta, n), r1 = rec_after(ta, n + 1);
+            if (T == 2) {
+                HIP_TRY(launch_step2(variant, desc, nshots, ntiles, ctx->grid_cap, kza, n, cur,
+                                     cur ^ 1, r0 >= 0, band_w, pend0, pend_state, max_ng,
+                                     ctx->stream));
+                pend0 = r0;
+                pend1 = -1;
+                pend_state = r1;
+            } else {
+                const int r2 = rec_after(ta, n + 2);
+                HIP_TRY(launch_step3(variant, desc, nshots, ntiles, ctx->grid_cap, kza, n, cur,
+                                     cur ^ 1, (r0 >= 0 ? 1 : 0) | (r1 >= 0 ? 2 : 0), band_w,
+                                     pend0, pend1, pend_state, max_ng, ctx->stream));
+                pend0 = r0;
+                pend1 = r1;
+                pend_state = r2;
+            }
             cur ^= 1;
-            last_recorded = true;
+            band_w ^= 1;
         }
         if (n < n1) {
-            HIP_TRY(launch_step(single, ctx->d_desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
-                                -1, ctx->stream));
-            cur ^= 1;
-            n++;
-            last_recorded = false;
+            // Fewer than T steps left: flush, then single steps (each kernel
+            // gathers the step before it).
+            HIP_TRY(launch_gather(desc, nshots, max_ng, band_w ^ 1, pend0, pend1, cur, pend_state,
+                                  ctx->stream));
+            pend0 = pend1 = -1;
+            for (bool first = true; n < n1; n++, first = false) {
+                HIP_TRY(launch_step(single, desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
+                                    first ? -1 : rec_slot_for_kernel(ta, n), ctx->stream));
+                cur ^= 1;
+            }
+            pend_state = rec_after(ta, n1 - 1);
         }
+        HIP_TRY(launch_gather(desc, nshots, max_ng, band_w ^ 1, pend0, pend1, cur, pend_state,
+                              ctx->stream));
     } else {
         for (; n < n1; n++) {
-            HIP_TRY(launch_step(variant, ctx->d_desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
-                                rec_slot_for_kernel(ctx->ta, n), ctx->stream));
+            HIP_TRY(launch_step(variant, desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
+                                rec_slot_for_kernel(ta, n), ctx->stream));
             cur ^= 1;
         }
-    }
-    if (n1 == ctx->ta.fd_nt && !last_recorded) {
-        const int slot = rec_slot_for_kernel(ctx->ta, n1);
-        if (slot >= 0 && max_ng > 0)
-            HIP_TRY(launch_record(ctx->d_desc, nshots, max_ng, cur, slot, ctx->stream));
+        if (n1 == ta.fd_nt) {
+            const int slot = rec_slot_for_kernel(ta, n1);
+            if (slot >= 0 && max_ng > 0)
+                HIP_TRY(launch_record(desc, nshots, max_ng, cur, slot, ctx->stream));
+        }
     }
     if (end_slot) *end_slot = cur;
     return SAM2D_OK;

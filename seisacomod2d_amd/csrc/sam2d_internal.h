@@ -53,11 +53,11 @@ struct ShotBuffers {
     float *amp = nullptr;
     int *gcell = nullptr;
     float *seis = nullptr;
-    int *rcolptr = nullptr, *rrow = nullptr, *rid = nullptr;   // receivers by column
-    size_t cells = 0, cap_state = 0, cap_seis = 0;
-    int cap_nts = 0, cap_ng = 0, cap_cols = 0, cap_csr = 0;
+    float *band = nullptr;         // [2][band_h][cells / nnz] receiver-row pressure
+    size_t cells = 0, cap_state = 0, cap_seis = 0, cap_band = 0;
+    int cap_nts = 0, cap_ng = 0;
     int reserve(size_t cells, int nslots, int nts, int ng, size_t nseis);
-    int reserve_csr(int ncols, int ng);
+    int reserve_band(size_t n);
     float *slot(int s) const { return state + (size_t)s * 4 * cells; }
     void release();
     ~ShotBuffers();
@@ -123,6 +123,9 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
                 int nslots = 2);
 int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int n0, int n1, int max_ng,
                   int start_slot, int *end_slot);
+// The same on an explicit device descriptor array (tile shapes of `variant`).
+int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int nshots, int ntiles, int variant,
+                     int n0, int n1, int max_ng, int start_slot, int *end_slot);
 void tile_shape(int nnz, int nnx, int waves, int chunk, int own_rows, int &ngroups, int &nchunks);
 }  // namespace sam2d
 

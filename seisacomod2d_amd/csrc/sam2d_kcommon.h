@@ -37,6 +37,31 @@ __device__ __forceinline__ SAM2D_GLOBAL T *gp(T *p) { return (SAM2D_GLOBAL T *)p
 __device__ __forceinline__ float shfl_up(float v, int d) { return __shfl_up(v, d, 64); }
 __device__ __forceinline__ float shfl_dn(float v, int d) { return __shfl_down(v, d, 64); }
 
+// Whole-wavefront lane shifts by DPP (wave_shr:1 = 0x138, wave_shl:1 = 0x130):
+// lane i receives lane i-1 (up) or lane i+1 (down); the lane without a source
+// receives 0 (bound_ctrl).  One VALU operation, no LDS; all 64 lanes must be
+// active.
+__device__ __forceinline__ float lane_up1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float lane_dn1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+
+// A register copy the compiler cannot merge away.  Taking a prefetched operand
+// into a sliding window through it leaves the prefetch register free for the
+// next column's load, so the load can be issued into it directly instead of
+// into a fresh register that the loop back edge would have to copy (a copy
+// that waits for the load to land).
+__device__ __forceinline__ float keep(float x)
+{
+    float y;
+    asm("v_mov_b32 %0, %1" : "=&v"(y) : "v"(x));
+    return y;
+}
+
 // Virtual block id -> tile id.  Blocks are dispatched round-robin over the 8
 // XCDs (block b lands on XCD b % 8); giving each XCD a contiguous range of
 // tiles keeps z-neighbouring tiles, which read each other's halo rows, in
@@ -44,6 +69,44 @@ __device__ __forceinline__ float shfl_dn(float v, int d) { return __shfl_down(v,
 __device__ __forceinline__ int xcd_tile(int v, int per_xcd)
 {
     return (v & 7) * per_xcd + (v >> 3);
+}
+
+// Band buffer of the receiver rows: [2 parities][2 slices][band_h][nnx].
+// Slice j of parity w holds the pressure after the (j+1)-th step of a
+// multi-step launch; the pointer is offset so that it is indexed by the
+// padded row: band_slice(d, w, j)[row * nnx + col].
+__device__ __forceinline__ SAM2D_GLOBAL float *band_slice(const ShotDev &d, int w, int j)
+{
+    return gp(d.band) + ((ptrdiff_t)(w * 2 + j) * d.band_h - d.rrow_lo) * (ptrdiff_t)d.nnx;
+}
+
+// Deferred receiver gather of the previous multi-step launch (cpu_record,
+// src/src_cpu/cpu_modelling_kernels.cpp:24-31, into the trace-major gather as
+// after cpu_transpose, :370-381): samples s0 / s1 from band slices 0 / 1 of
+// parity band_r, sample ss as px + pz of the state slot (each skipped when
+// < 0).
+__device__ __forceinline__ void gather_deferred(const ShotDev *sh, int nshots, int band_r, int s0,
+                                                int s1, int state_slot, int ss, int gtid,
+                                                int gstride)
+{
+    for (int s = 0; s < nshots; ++s) {
+        const ShotDev &d = sh[s];
+        const SAM2D_GLOBAL float *px = gp(slot_field(d, state_slot, 0));
+        const SAM2D_GLOBAL float *pz = gp(slot_field(d, state_slot, 1));
+        const SAM2D_GLOBAL int *gc = gp(d.gcell);
+        SAM2D_GLOBAL float *seis = gp(d.seis);
+        for (int g = gtid; g < d.ng; g += gstride) {
+            const int c = gc[g];
+            SAM2D_GLOBAL float *tr = seis + (size_t)g * d.real_nt;
+            if (s0 >= 0 || s1 >= 0) {
+                const int col = c / d.nnz, row = c - col * d.nnz;
+                const size_t at = (size_t)row * d.nnx + col;
+                if (s0 >= 0) tr[s0] = band_slice(d, band_r, 0)[at];
+                if (s1 >= 0) tr[s1] = band_slice(d, band_r, 1)[at];
+            }
+            if (ss >= 0) tr[ss] = px[c] + pz[c];
+        }
+    }
 }
 
 // Shot owning a flat tile id (shots are few; descriptors are uniform).

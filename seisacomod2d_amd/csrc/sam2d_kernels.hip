@@ -243,18 +243,21 @@ static const int kNTS[] = {0, 1};
 static const int nW = 4, nC = 5, nP = 2, nN = 2;
 static const int kChunks2[] = {32, 64, 128, 256, 512};
 static const int nSingle = nW * nC * nP * nN;     // single-step variants first
-static const int nDouble = nW * 5;                 // then two-step variants
+static const int nDouble = nW * 5 * nP;            // then two-step variants
+static const int nTriple = nW * 5 * nP;            // then three-step variants
 
-int num_variants() { return nSingle + nDouble; }
+int num_variants() { return nSingle + nDouble + nTriple; }
 
 // Variant index -> (steps per launch, waves, chunk, prefetch, nt-stores).
 static void decode(int v, int &w, int &c, int &pf, int &nts, int &steps)
 {
     if (v >= nSingle) {
         v -= nSingle;
-        steps = 2; pf = 2; nts = 0;
+        steps = 2; nts = 0;
+        if (v >= nDouble) { v -= nDouble; steps = 3; }
         c = kChunks2[v % 5];
-        w = kWaves[v / 5];
+        w = kWaves[(v / 5) % nW];
+        pf = kPF[v / (5 * nW)];
         return;
     }
     steps = 1;
@@ -275,6 +278,12 @@ void variant_shape(int v, int *waves, int *chunk)
     decode(v, *waves, *chunk, pf, nts);
 }
 
+void variant_shape_pf(int v, int *waves, int *chunk, int *pf)
+{
+    int nts;
+    decode(v, *waves, *chunk, *pf, nts);
+}
+
 int variant_steps(int v)
 {
     int w, c, pf, nts, steps;
@@ -282,7 +291,11 @@ int variant_steps(int v)
     return steps;
 }
 
-int variant_own_rows(int v) { return variant_steps(v) == 2 ? OWN_Z2 : OWN_Z; }
+int variant_own_rows(int v)
+{
+    const int t = variant_steps(v);
+    return t == 3 ? OWN_Z3 : (t == 2 ? OWN_Z2 : OWN_Z);
+}
 
 int variant_single_step(int v)
 {
@@ -299,7 +312,9 @@ void variant_name(int v, char *buf, int n)
 {
     int w, c, pf, nts, steps;
     decode(v, w, c, pf, nts, steps);
-    if (steps == 2) snprintf(buf, n, "twostep_waves%d_chunk%d", w, c);
+    if (steps > 1)
+        snprintf(buf, n, "%s_waves%d_chunk%d%s", steps == 3 ? "threestep" : "twostep", w, c,
+                 pf == 4 ? "_pf4" : "");
     else snprintf(buf, n, "waves%d_chunk%d_pf%d%s", w, c, pf, nts ? "_nts" : "");
 }
 

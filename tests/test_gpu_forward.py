@@ -188,31 +188,35 @@ def test_errors_are_statuses():
         sam.Modeller(_grid(g, fdop=2), g["vp"], g["rho"])
 
 
-def _twostep_variants():
-    return [i for i, n in enumerate(sam.variants()) if n.startswith("twostep")]
+def _multistep_variants():
+    return [i for i, n in enumerate(sam.variants()) if n.startswith(("twostep", "threestep"))]
 
 
 @pytest.mark.parametrize("case", ["c0_twolayer_npml32", "rect_dtfactor", "salt_small"])
-def test_twostep_kernel_bitwise_equals_single_step(case):
-    # Two time steps per pass (intermediate state kept in registers, receivers
-    # recorded in place) must reproduce the single-step kernel bit for bit.
+def test_multistep_kernels_bitwise_equal_single_step(case):
+    # Two or three time steps per pass (intermediate states kept in registers,
+    # receivers of the intermediate steps kept in the band buffer) must
+    # reproduce the single-step kernel bit for bit, including the tail steps
+    # when fd_nt is not a multiple of the steps per pass.
     g = load_golden(case)
     with sam.Modeller(_grid(g), g["vp"], g["rho"]) as m:
         m.set_option("variant", 11)                    # a single-step variant
         ref = m.forward([_shot(g)])[0]
-        for v in _twostep_variants()[1::2]:
+        for v in _multistep_variants()[1::2]:
             m.set_option("variant", v)
             s = m.forward([_shot(g)])[0]
             assert np.array_equal(s.view(np.int32), ref.view(np.int32)), sam.variants()[v]
     assert_seis_close(ref[g["trace_idx"]], g["seis"], TOL, case)
 
 
-def test_twostep_batched_windows_bitwise():
+def test_multistep_batched_windows_bitwise():
     g, shots, wins = _windows_case()
+    names = sam.variants()
     with sam.Modeller(_grid(g, npml=24), g["vp"], g["rho"]) as m:
         m.set_option("variant", 11)
         ref = m.forward_windows(shots, wins)
-        m.set_option("variant", _twostep_variants()[2])
-        out = m.forward_batched(shots, wins)
-    for a, b in zip(ref, out):
-        assert np.array_equal(a.view(np.int32), b.view(np.int32))
+        for name in ("twostep_waves1_chunk128", "threestep_waves2_chunk64_pf4"):
+            m.set_option("variant", names.index(name))
+            out = m.forward_batched(shots, wins)
+            for a, b in zip(ref, out):
+                assert np.array_equal(a.view(np.int32), b.view(np.int32)), name

@@ -37,8 +37,9 @@ def test_version_and_variants():
     assert b"gfx950" in L.sam2d_version()
     v = sam.variants()
     assert len(v) == L.sam2d_num_variants() >= 4
-    assert all(name.startswith(("waves", "twostep_waves")) for name in v)
+    assert all(name.startswith(("waves", "twostep_waves", "threestep_waves")) for name in v)
     assert any(name.startswith("twostep") for name in v)
+    assert any(name.startswith("threestep") for name in v)
     assert len(set(v)) == len(v)
 
 
