@@ -240,7 +240,8 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
         SAM2D_TRY(upload(b.gcell, p.gcell.data(), sizeof(int) * p.ng, ctx->stream));
         // Rows spanned by the receivers: the multi-step kernels store the
         // intermediate pressure of these rows into a band buffer (two
-        // parities x two slices of band_h x nnx) for the deferred gather.
+        // parities x BAND_SLICES slices of band_h x nnx) for the deferred
+        // gather.
         int rlo = 1, rhi = 0;
         for (int g = 0; g < p.ng; g++) {
             const int row = p.gcell[g] % nnz;
@@ -248,7 +249,7 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
             if (g == 0 || row > rhi) rhi = row;
         }
         const int band_h = p.ng > 0 ? rhi - rlo + 1 : 0;
-        SAM2D_TRY(b.reserve_band((size_t)4 * band_h * nnx));
+        SAM2D_TRY(b.reserve_band((size_t)2 * BAND_SLICES * band_h * nnx));
         HIP_TRY(launch_fill(b.slot(0), 4 * N, 0.0f, ctx->stream));
         if (p.ng) HIP_TRY(launch_fill(b.seis, (size_t)p.ng * real_nt, 0.0f, ctx->stream));
         ShotDev &d = desc[s];
@@ -307,7 +308,7 @@ static int rec_after(const TimeAxis &ta, int n)
 // start_slot; *end_slot receives the slot holding the final state.
 // Single-step variants record step n-1 inside kernel n (deferred gather), and
 // the receivers of the last step when n1 is the last step.  Multi-step
-// variants (T = 2 or 3 steps per launch) gather the receivers of each launch
+// variants (T = 2 to 5 steps per launch) gather the receivers of each launch
 // at the start of the next one (intermediate steps from the band buffer, the
 // last from the stored state); fewer than T remaining steps run the
 // single-step kernel; a final gather launch records what is still pending.
@@ -328,36 +329,29 @@ int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int nshots, int ntiles
     if (T > 1) {
         const int single = variant_single_step(variant);
         const bool kza = ta.dtdx == ta.dtdz;
-        // Samples still to gather: band slices 0 / 1 of parity band_w ^ 1,
-        // and the current state.
-        int pend0 = -1, pend1 = -1, pend_state = -1, band_w = 0;
+        // Samples still to gather: band slices of parity band_w ^ 1, and the
+        // current state.
+        int pend[BAND_SLICES] = {-1, -1, -1, -1};
+        int pend_state = -1, band_w = 0;
         for (; n + T - 1 < n1; n += T) {
-            const int r0 = rec_after(ta, n), r1 = rec_after(ta, n + 1);
-            if (T == 2) {
-                HIP_TRY(launch_step2(variant, desc, nshots, ntiles, ctx->grid_cap, kza, n, cur,
-                                     cur ^ 1, r0 >= 0, band_w, pend0, pend_state, max_ng,
-                                     ctx->stream));
-                pend0 = r0;
-                pend1 = -1;
-                pend_state = r1;
-            } else {
-                const int r2 = rec_after(ta, n + 2);
-                HIP_TRY(launch_step3(variant, desc, nshots, ntiles, ctx->grid_cap, kza, n, cur,
-                                     cur ^ 1, (r0 >= 0 ? 1 : 0) | (r1 >= 0 ? 2 : 0), band_w,
-                                     pend0, pend1, pend_state, max_ng, ctx->stream));
-                pend0 = r0;
-                pend1 = r1;
-                pend_state = r2;
+            int mask = 0, rec[BAND_SLICES] = {-1, -1, -1, -1};
+            for (int j = 0; j < T - 1; j++) {
+                rec[j] = rec_after(ta, n + j);
+                if (rec[j] >= 0) mask |= 1 << j;
             }
+            HIP_TRY(launch_stepn(variant, desc, nshots, ntiles, ctx->grid_cap, kza, n, cur, cur ^ 1,
+                                 mask, band_w, pend, pend_state, max_ng, ctx->stream));
+            for (int j = 0; j < BAND_SLICES; j++) pend[j] = rec[j];
+            pend_state = rec_after(ta, n + T - 1);
             cur ^= 1;
             band_w ^= 1;
         }
         if (n < n1) {
             // Fewer than T steps left: flush, then single steps (each kernel
             // gathers the step before it).
-            HIP_TRY(launch_gather(desc, nshots, max_ng, band_w ^ 1, pend0, pend1, cur, pend_state,
+            HIP_TRY(launch_gather(desc, nshots, max_ng, band_w ^ 1, pend, cur, pend_state,
                                   ctx->stream));
-            pend0 = pend1 = -1;
+            for (int j = 0; j < BAND_SLICES; j++) pend[j] = -1;
             for (bool first = true; n < n1; n++, first = false) {
                 HIP_TRY(launch_step(single, desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
                                     first ? -1 : rec_slot_for_kernel(ta, n), ctx->stream));
@@ -365,7 +359,7 @@ int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int nshots, int ntiles
             }
             pend_state = rec_after(ta, n1 - 1);
         }
-        HIP_TRY(launch_gather(desc, nshots, max_ng, band_w ^ 1, pend0, pend1, cur, pend_state,
+        HIP_TRY(launch_gather(desc, nshots, max_ng, band_w ^ 1, pend, cur, pend_state,
                               ctx->stream));
     } else {
         for (; n < n1; n++) {

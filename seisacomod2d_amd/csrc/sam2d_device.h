@@ -9,13 +9,11 @@ namespace sam2d {
 constexpr int OWN_Z = 58;
 constexpr int HALO_T = 3;
 constexpr int HALO_B = 64 - OWN_Z - HALO_T;
-// The same for the kernel that advances two time steps per pass: the second
-// step's dependency cone needs six halo rows on each side.
-constexpr int OWN_Z2 = 52;
-constexpr int HALO_T2 = 6;
-// Three time steps per pass: nine halo rows on each side.
-constexpr int OWN_Z3 = 46;
-constexpr int HALO_T3 = 9;
+// Kernels that advance T > 1 time steps per pass carry 3T halo rows on each
+// side and own 64 - 6T rows.
+// Band slices of the receiver-row pressure buffer: one per intermediate step
+// of a launch (up to five steps per launch).
+constexpr int BAND_SLICES = 4;
 
 // One resident shot: its padded domain, coefficient fields, ping-pong state
 // and receiver list, all device pointers.  Kernels read it from device memory
@@ -41,8 +39,8 @@ struct ShotDev {
     size_t ncell;          //   field f of slot s at state + (4*s + f)*ncell
     const int *gcell;      // [ng] receiver cells (ix*nnz + iz)
     float *seis;           // [ng][real_nt]
-    float *band;           // [2][2][band_h][nnx]: pressure of the receiver rows
-                           // after the intermediate steps of a multi-step launch
+    float *band;           // [2][BAND_SLICES][band_h][nnx]: pressure of the receiver
+                           // rows after the intermediate steps of a launch
 };
 
 __host__ __device__ inline float *slot_field(const ShotDev &d, int slot, int f)
@@ -83,35 +81,29 @@ void variant_shape(int v, int *waves, int *chunk);
 // The same plus the operand columns in flight (prefetch depth).
 void variant_shape_pf(int v, int *waves, int *chunk, int *pf);
 void variant_name(int v, char *buf, int n);
-// Time steps one launch of variant v advances (1 or 2), and the rows a
+// Time steps one launch of variant v advances (1 to 5), and the rows a
 // wavefront owns for it.
 int variant_steps(int v);
 int variant_own_rows(int v);
 // The single-step variant with the same workgroup and chunk shape.
 int variant_single_step(int v);
 
-// Two time steps (step, step+1) of every staged shot: in_slot -> out_slot.
-// band_store: keep the pressure after step `step` of the receiver rows in
-// band parity band_w.  Before that, the launch gathers the receivers of the
-// previous launch: sample band_sample from band parity band_w ^ 1 and sample
-// state_sample from the in_slot state (each skipped when < 0).
-// kz_alias: every staged shot's KZ is its KX (dt/dz == dt/dx).
-hipError_t launch_step2(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
-                        bool kz_alias, int step, int in_slot, int out_slot, bool band_store,
-                        int band_w, int band_sample, int state_sample, int max_ng,
-                        hipStream_t stream);
-// Three time steps (step .. step+2): band_mask bit j keeps the pressure after
-// step step+j in band slice j of parity band_w; the previous launch's samples
-// are gathered first (band slices 0 / 1 of parity band_w ^ 1, the in_slot
-// state), each skipped when < 0.
-hipError_t launch_step3(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
+// T = variant_steps(variant) time steps (step .. step+T-1) of every staged
+// shot, in_slot -> out_slot.  band_mask bit j keeps the pressure after step
+// step+j (j < T-1) of the receiver rows in band slice j of parity band_w.
+// Before that, the launch gathers the previous launch's samples:
+// band_samples[j] from slice j of parity band_w ^ 1 and state_sample from the
+// in_slot state (each skipped when < 0).  kz_alias: every staged shot's KZ is
+// its KX (dt/dz == dt/dx).
+hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
                         bool kz_alias, int step, int in_slot, int out_slot, int band_mask,
-                        int band_w, int band_s0, int band_s1, int state_sample, int max_ng,
+                        int band_w, const int *band_samples, int state_sample, int max_ng,
                         hipStream_t stream);
 // The deferred gathers as a launch of their own (after the last multi-step
-// launch): band slices 0 / 1 of parity band_r, and the state slot.
-hipError_t launch_gather(const ShotDev *d_shots, int nshots, int max_ng, int band_r, int band_s0,
-                         int band_s1, int state_slot, int state_sample, hipStream_t stream);
+// launch): band_samples[j] from slice j of parity band_r, the state slot.
+hipError_t launch_gather(const ShotDev *d_shots, int nshots, int max_ng, int band_r,
+                         const int *band_samples, int state_slot, int state_sample,
+                         hipStream_t stream);
 
 // One time step of every staged shot: reads state slot in_slot, writes
 // out_slot; rec_sample >= 0 gathers the receivers of the in_slot state.

@@ -71,24 +71,25 @@ __device__ __forceinline__ int xcd_tile(int v, int per_xcd)
     return (v & 7) * per_xcd + (v >> 3);
 }
 
-// Band buffer of the receiver rows: [2 parities][2 slices][band_h][nnx].
+// Band buffer of the receiver rows: [2 parities][BAND_SLICES][band_h][nnx].
 // Slice j of parity w holds the pressure after the (j+1)-th step of a
 // multi-step launch; the pointer is offset so that it is indexed by the
 // padded row: band_slice(d, w, j)[row * nnx + col].
 __device__ __forceinline__ SAM2D_GLOBAL float *band_slice(const ShotDev &d, int w, int j)
 {
-    return gp(d.band) + ((ptrdiff_t)(w * 2 + j) * d.band_h - d.rrow_lo) * (ptrdiff_t)d.nnx;
+    return gp(d.band) + ((ptrdiff_t)(w * BAND_SLICES + j) * d.band_h - d.rrow_lo) *
+                            (ptrdiff_t)d.nnx;
 }
 
 // Deferred receiver gather of the previous multi-step launch (cpu_record,
 // src/src_cpu/cpu_modelling_kernels.cpp:24-31, into the trace-major gather as
-// after cpu_transpose, :370-381): samples s0 / s1 from band slices 0 / 1 of
-// parity band_r, sample ss as px + pz of the state slot (each skipped when
+// after cpu_transpose, :370-381): samples bs.x .. bs.w from band slices 0..3
+// of parity band_r, sample ss as px + pz of the state slot (each skipped when
 // < 0).
-__device__ __forceinline__ void gather_deferred(const ShotDev *sh, int nshots, int band_r, int s0,
-                                                int s1, int state_slot, int ss, int gtid,
-                                                int gstride)
+__device__ __forceinline__ void gather_deferred(const ShotDev *sh, int nshots, int band_r, int4 bs,
+                                                int state_slot, int ss, int gtid, int gstride)
 {
+    const int smp[BAND_SLICES] = {bs.x, bs.y, bs.z, bs.w};
     for (int s = 0; s < nshots; ++s) {
         const ShotDev &d = sh[s];
         const SAM2D_GLOBAL float *px = gp(slot_field(d, state_slot, 0));
@@ -98,12 +99,11 @@ __device__ __forceinline__ void gather_deferred(const ShotDev *sh, int nshots, i
         for (int g = gtid; g < d.ng; g += gstride) {
             const int c = gc[g];
             SAM2D_GLOBAL float *tr = seis + (size_t)g * d.real_nt;
-            if (s0 >= 0 || s1 >= 0) {
-                const int col = c / d.nnz, row = c - col * d.nnz;
-                const size_t at = (size_t)row * d.nnx + col;
-                if (s0 >= 0) tr[s0] = band_slice(d, band_r, 0)[at];
-                if (s1 >= 0) tr[s1] = band_slice(d, band_r, 1)[at];
-            }
+            const int col = c / d.nnz, row = c - col * d.nnz;
+            const size_t at = (size_t)row * d.nnx + col;
+            #pragma unroll
+            for (int j = 0; j < BAND_SLICES; j++)
+                if (smp[j] >= 0) tr[smp[j]] = band_slice(d, band_r, j)[at];
             if (ss >= 0) tr[ss] = px[c] + pz[c];
         }
     }

@@ -243,18 +243,19 @@ static const int kNTS[] = {0, 1};
 static const int nW = 4, nC = 5, nP = 2, nN = 2;
 static const int kChunks2[] = {32, 64, 128, 256, 512};
 static const int nSingle = nW * nC * nP * nN;     // single-step variants first
-static const int nDouble = nW * 5 * nP;            // then two-step variants
-static const int nTriple = nW * 5 * nP;            // then three-step variants
+static const int nMultiEach = nW * 5 * nP;        // then 2-, 3-, 4- and 5-step variants
+static const int kMaxSteps = 5;
 
-int num_variants() { return nSingle + nDouble + nTriple; }
+int num_variants() { return nSingle + nMultiEach * (kMaxSteps - 1); }
 
 // Variant index -> (steps per launch, waves, chunk, prefetch, nt-stores).
 static void decode(int v, int &w, int &c, int &pf, int &nts, int &steps)
 {
     if (v >= nSingle) {
         v -= nSingle;
-        steps = 2; nts = 0;
-        if (v >= nDouble) { v -= nDouble; steps = 3; }
+        steps = 2 + v / nMultiEach;
+        v %= nMultiEach;
+        nts = 0;
         c = kChunks2[v % 5];
         w = kWaves[(v / 5) % nW];
         pf = kPF[v / (5 * nW)];
@@ -294,7 +295,7 @@ int variant_steps(int v)
 int variant_own_rows(int v)
 {
     const int t = variant_steps(v);
-    return t == 3 ? OWN_Z3 : (t == 2 ? OWN_Z2 : OWN_Z);
+    return t == 1 ? OWN_Z : 64 - 6 * t;
 }
 
 int variant_single_step(int v)
@@ -312,9 +313,9 @@ void variant_name(int v, char *buf, int n)
 {
     int w, c, pf, nts, steps;
     decode(v, w, c, pf, nts, steps);
+    static const char *kStepNames[] = {"", "", "twostep", "threestep", "fourstep", "fivestep"};
     if (steps > 1)
-        snprintf(buf, n, "%s_waves%d_chunk%d%s", steps == 3 ? "threestep" : "twostep", w, c,
-                 pf == 4 ? "_pf4" : "");
+        snprintf(buf, n, "%s_waves%d_chunk%d%s", kStepNames[steps], w, c, pf == 4 ? "_pf4" : "");
     else snprintf(buf, n, "waves%d_chunk%d_pf%d%s", w, c, pf, nts ? "_nts" : "");
 }
 

@@ -189,12 +189,13 @@ def test_errors_are_statuses():
 
 
 def _multistep_variants():
-    return [i for i, n in enumerate(sam.variants()) if n.startswith(("twostep", "threestep"))]
+    return [i for i, n in enumerate(sam.variants())
+            if n.startswith(("twostep", "threestep", "fourstep", "fivestep"))]
 
 
 @pytest.mark.parametrize("case", ["c0_twolayer_npml32", "rect_dtfactor", "salt_small"])
 def test_multistep_kernels_bitwise_equal_single_step(case):
-    # Two or three time steps per pass (intermediate states kept in registers,
+    # Two to five time steps per pass (intermediate states kept in registers,
     # receivers of the intermediate steps kept in the band buffer) must
     # reproduce the single-step kernel bit for bit, including the tail steps
     # when fd_nt is not a multiple of the steps per pass.
@@ -215,7 +216,8 @@ def test_multistep_batched_windows_bitwise():
     with sam.Modeller(_grid(g, npml=24), g["vp"], g["rho"]) as m:
         m.set_option("variant", 11)
         ref = m.forward_windows(shots, wins)
-        for name in ("twostep_waves1_chunk128", "threestep_waves2_chunk64_pf4"):
+        for name in ("twostep_waves1_chunk128", "threestep_waves2_chunk64_pf4",
+                     "fourstep_waves4_chunk32", "fivestep_waves1_chunk256_pf4"):
             m.set_option("variant", names.index(name))
             out = m.forward_batched(shots, wins)
             for a, b in zip(ref, out):

@@ -37,9 +37,10 @@ def test_version_and_variants():
     assert b"gfx950" in L.sam2d_version()
     v = sam.variants()
     assert len(v) == L.sam2d_num_variants() >= 4
-    assert all(name.startswith(("waves", "twostep_waves", "threestep_waves")) for name in v)
-    assert any(name.startswith("twostep") for name in v)
-    assert any(name.startswith("threestep") for name in v)
+    prefixes = ("twostep", "threestep", "fourstep", "fivestep")
+    assert all(name.startswith(("waves",) + tuple(p + "_waves" for p in prefixes)) for name in v)
+    for p in prefixes:
+        assert any(name.startswith(p) for name in v), p
     assert len(set(v)) == len(v)
 
 
