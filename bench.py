@@ -311,14 +311,18 @@ def anomaly_model(nx=1201, nz=601, h=10.0, seed=3):
     return true.astype(np.float32), bg.astype(np.float32), rho
 
 
-def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label):
+def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False):
     """Gradient time against the forward modelling of the same shots: the
     batched forward (autotuned, the fastest forward path) and the per-shot
     loop."""
     res = []
     with sam.Modeller(grid, vtrue, rho) as mt:
+        if no_graph:
+            mt.set_option("graph", 0)
         obs = mt.forward_batched(sh, wins)
     with sam.Modeller(grid, v0, rho) as m:
+        if no_graph:
+            m.set_option("graph", 0)
         m.autotune(windows=wins, nshots=len(sh), steps=6)
         d = m.forward_batched(sh, wins)                          # warm
         t0 = time.perf_counter()
@@ -354,7 +358,8 @@ def bench_gradient(args, sam):
     grid = sam.Grid(1201, 601, 10.0, 10.0, 3.0, 0.001, 0.001, 10.0, npml=32)
     rec = np.stack([np.arange(1201) * 10.0, np.full(1201, 10.0)], 1).astype(np.float32)
     sh = [sam.Shot(10.0 * int((k + 0.5) * 1201 / n3), 10.0, rec) for k in range(n3)]
-    out += _grad_case(sam, grid, vtrue, v0, rho, sh, None, [100], "config3_gradient_1201x601_nt3000")
+    out += _grad_case(sam, grid, vtrue, v0, rho, sh, None, [100], "config3_gradient_1201x601_nt3000",
+                      args.no_graph)
     # config 4: salt model, smoothed starting model, windowed shots
     vs, rs = salt_model()
     from scipy.ndimage import gaussian_filter
@@ -363,7 +368,7 @@ def bench_gradient(args, sam):
     grid4 = sam.Grid(3201, 1201, 7.62, 7.62, 4.0, 0.0005, 0.0005, 15.0, npml=60)
     sh4, w4 = salt_shots(sam, n4, seed=11)
     out += _grad_case(sam, grid4, vs, v4, rs, sh4, w4, [50, 100, 200] if not args.quick else [200],
-                      "config4_gradient_3201x1201_nt8000_windows")
+                      "config4_gradient_3201x1201_nt8000_windows", args.no_graph)
     print(json.dumps({"metric": METRIC,
                       "unit": "s/shot", "higher_is_better": False, "dtype": "f32",
                       "results": out}), flush=True)

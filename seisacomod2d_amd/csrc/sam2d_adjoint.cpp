@@ -24,13 +24,13 @@ struct AdjShotBuffers {
     float *lam = nullptr;     // 2 adjoint slots x 4 fields, then gk, gbx, gbz
     float *ckpt = nullptr;    // nckpt slots x 4 fields
     float *srcg = nullptr;
-    int *colptr = nullptr, *rrow = nullptr, *rid = nullptr;
+    int *rcell = nullptr, *rptr = nullptr, *rid = nullptr;
     float *res = nullptr;
     size_t cap[7] = {0, 0, 0, 0, 0, 0, 0};   // bytes of each buffer above
     ~AdjShotBuffers()
     {
         dev_free(lam); dev_free(ckpt); dev_free(srcg);
-        dev_free(colptr); dev_free(rrow); dev_free(rid); dev_free(res);
+        dev_free(rcell); dev_free(rptr); dev_free(rid); dev_free(res);
     }
 };
 
@@ -125,6 +125,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     SAM2D_TRY(stage_batch(ctx, plans, variant, &ntiles, k + 1));
     if ((int)A.bufs.size() < ns) A.bufs.resize(ns);
     std::vector<AdjDev> desc(ns);
+    std::vector<int> ncells(ns, 0);
     const int nseg = (ta.fd_nt + k - 1) / k;
     for (int s = 0; s < ns; s++) {
         ShotPlan &p = plans[s];
@@ -136,8 +137,8 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         SAM2D_TRY(grow((void **)&b.lam, b.cap[0], 11 * N * sizeof(float)));
         SAM2D_TRY(grow((void **)&b.ckpt, b.cap[1], (size_t)nseg * 4 * N * sizeof(float)));
         SAM2D_TRY(grow((void **)&b.srcg, b.cap[2], (size_t)std::max(1, ta.nts) * sizeof(float)));
-        SAM2D_TRY(grow((void **)&b.colptr, b.cap[3], (size_t)(h.nnx + 1) * sizeof(int)));
-        SAM2D_TRY(grow((void **)&b.rrow, b.cap[4], ng1 * sizeof(int)));
+        SAM2D_TRY(grow((void **)&b.rcell, b.cap[3], ng1 * sizeof(int)));
+        SAM2D_TRY(grow((void **)&b.rptr, b.cap[4], (ng1 + 1) * sizeof(int)));
         SAM2D_TRY(grow((void **)&b.rid, b.cap[5], ng1 * sizeof(int)));
         SAM2D_TRY(grow((void **)&b.res, b.cap[6], ng1 * ta.real_nt * sizeof(float)));
         float *lam = b.lam, *grad = b.lam + 8 * N;
@@ -148,19 +149,27 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         const double area = (double)(ctx->grid.dx * ctx->grid.dz);
         for (int n = 0; n < ta.nts; n++)
             srcg[n] = (float)((double)ctx->grid.fd_dt * ctx->wavelet[n] / area);
-        std::vector<int> colptr(h.nnx + 1, 0), rrow(std::max(1, p.ng)), rid(std::max(1, p.ng));
-        for (int g = 0; g < p.ng; g++) colptr[p.gcell[g] / h.nnz + 1]++;
-        for (int c = 0; c < h.nnx; c++) colptr[c + 1] += colptr[c];
-        std::vector<int> fillp(colptr.begin(), colptr.end() - 1);
-        for (int g = 0; g < p.ng; g++) {
-            const int c = p.gcell[g] / h.nnz;
-            const int e = fillp[c]++;
-            rrow[e] = p.gcell[g] % h.nnz;
+        // Receivers grouped by cell (ascending cell, receiver order kept
+        // within a cell): duplicates of one cell are summed in a fixed order.
+        std::vector<int> order(p.ng);
+        for (int g = 0; g < p.ng; g++) order[g] = g;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return p.gcell[a] < p.gcell[b]; });
+        std::vector<int> rcell, rptr(1, 0), rid(std::max(1, p.ng));
+        for (int e = 0; e < p.ng; e++) {
+            const int g = order[e];
+            if (rcell.empty() || rcell.back() != p.gcell[g]) {
+                if (!rcell.empty()) rptr.push_back(e);
+                rcell.push_back(p.gcell[g]);
+            }
             rid[e] = g;
         }
+        rptr.push_back(p.ng);
+        if (rcell.empty()) rptr.assign(1, 0);
+        ncells[s] = (int)rcell.size();
         SAM2D_TRY(upload(b.srcg, srcg.data(), srcg.size() * 4, ctx->stream));
-        SAM2D_TRY(upload(b.colptr, colptr.data(), colptr.size() * 4, ctx->stream));
-        SAM2D_TRY(upload(b.rrow, rrow.data(), (size_t)p.ng * 4, ctx->stream));
+        SAM2D_TRY(upload(b.rcell, rcell.data(), rcell.size() * 4, ctx->stream));
+        SAM2D_TRY(upload(b.rptr, rptr.data(), rptr.size() * 4, ctx->stream));
         SAM2D_TRY(upload(b.rid, rid.data(), (size_t)p.ng * 4, ctx->stream));
         SAM2D_TRY(upload(b.res, adj[s], (size_t)p.ng * ta.real_nt * 4, ctx->stream));
 
@@ -176,7 +185,9 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         d.prof_x = fd.prof_x; d.prof_z = fd.prof_z;
         d.fwd = fd.state; d.lam = lam; d.ncell = N;
         d.gk = grad; d.gbx = grad + N; d.gbz = grad + 2 * N;
-        d.colptr = b.colptr; d.rrow = b.rrow; d.rid = b.rid; d.res = b.res;
+        d.xin_lo = fd.xin_lo; d.xin_hi = fd.xin_hi;
+        d.ncells = ncells[s];
+        d.rcell = b.rcell; d.rptr = b.rptr; d.rid = b.rid; d.res = b.res;
     }
     if (ns > A.cap_desc) {
         dev_free(A.d_desc); A.d_desc = nullptr; A.cap_desc = 0;
@@ -184,8 +195,10 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         A.cap_desc = ns;
     }
     SAM2D_TRY(upload(A.d_desc, desc.data(), sizeof(AdjDev) * ns, ctx->stream));
-    int max_ng = 0;
+    int max_ng = 0, max_cells = 0;
     for (auto &p : plans) max_ng = std::max(max_ng, p.ng);
+    for (int c : ncells) max_cells = std::max(max_cells, c);
+    const bool kza = ta.dtdx == ta.dtdz;
 
     auto save_or_restore = [&](int m, int slot, bool save) -> int {
         for (int s = 0; s < ns; s++) {
@@ -199,13 +212,13 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     };
 
     // Forward-sweep kernel: the context's variant when it advances T > 1
-    // steps per launch and k is a multiple of T (every checkpoint then falls
-    // on a launch boundary), on descriptors with that variant's tile shapes;
-    // otherwise the single-step variant of the recomputation.
+    // steps per launch, on descriptors with that variant's tile shapes (each
+    // checkpoint segment runs its steps as T-step launches plus single steps
+    // for the rest); otherwise the single-step variant of the recomputation.
     int fwd_variant = pick_variant(ctx);
     const ShotDev *fwd_desc = ctx->d_desc;
     int fwd_tiles = ntiles;
-    if (variant_steps(fwd_variant) > 1 && k % variant_steps(fwd_variant) == 0) {
+    if (variant_steps(fwd_variant) > 1) {
         int w2, c2;
         variant_shape(fwd_variant, &w2, &c2);
         std::vector<ShotDev> d2(ctx->staged_desc.begin(), ctx->staged_desc.begin() + ns);
@@ -244,12 +257,21 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     for (int m = nseg - 1; m >= 0; m--) {
         const int n0 = m * k, n1 = std::min(ta.fd_nt, n0 + k);
         SAM2D_TRY(save_or_restore(m, 0, false));
-        for (int n = n0; n < n1; n++)
+        // Recompute the segment's states into ring slots 1 .. n1-n0: three
+        // steps per launch with the ring form of a three-step variant, single
+        // steps for the rest.
+        int n = n0;
+        if (ring_supported(fwd_variant))
+            for (; n + 2 < n1; n += 3)
+                HIP_TRY(launch_stepn_ring(fwd_variant, fwd_desc, ns, fwd_tiles, ctx->grid_cap, kza,
+                                          n, n - n0, ctx->stream));
+        for (; n < n1; n++)
             HIP_TRY(launch_step(variant, ctx->d_desc, ns, ntiles, ctx->grid_cap, n, n - n0,
                                 n - n0 + 1, -1, ctx->stream));
         for (int n = n1 - 1; n >= n0; n--) {
-            HIP_TRY(launch_adjoint_step(variant, A.d_desc, ns, ntiles, n, lam, lam ^ 1, n - n0,
-                                        n - n0 + 1, rec_sample(ta, n), ctx->stream));
+            HIP_TRY(launch_inject(A.d_desc, ns, max_cells, lam, rec_sample(ta, n), ctx->stream));
+            HIP_TRY(launch_adjoint_step(variant, A.d_desc, ns, ntiles, kza, n, lam, lam ^ 1, n - n0,
+                                        n - n0 + 1, ctx->stream));
             lam ^= 1;
         }
     }

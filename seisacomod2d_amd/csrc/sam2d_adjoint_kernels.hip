@@ -1,4 +1,4 @@
-// sam2d_adjoint_kernels.hip -- gfx950 kernel of one backward (adjoint) step
+// sam2d_adjoint_kernels.hip -- gfx950 kernels of one backward (adjoint) step
 // of the staggered-grid scheme in sam2d_kernels.hip, with the FWI gradient
 // accumulation fused in.
 //
@@ -8,7 +8,8 @@
 // after step n (L_px, L_pz, L_vx, L_vz) and r_n the residual injected where
 // step n was recorded:
 //
-//   L_p'  = Dx+^T (BXe L_vx) + Dz+^T (BZe L_vz) + r_n           (v-update, record)
+//   L_px += r_n, L_pz += r_n at the receiver cells               (record, k_inject)
+//   L_p'  = Dx+^T (BXe L_vx) + Dz+^T (BZe L_vz)                   (v-update)
 //   L_px' = L_px + L_p',   L_pz' = L_pz + L_p'                   (p = px + pz)
 //   out:  L_px = dax L_px',  L_pz = daz L_pz'                     (p sponge)
 //         L_vx = dbx L_vx + Dx-^T (KXe L_px')                     (v sponge, p-update)
@@ -25,33 +26,63 @@
 //
 // The tiling, the halo recomputation, the x-march with mod-4 register windows
 // and the clamped loads follow the forward kernel: phase A (the L_p' part)
-// runs at column c, phase B (the velocity part) at column j = c - 2.
+// runs at column c, phase B (the velocity part) at column j = c - 2.  As in
+// the multi-step forward kernel, z-neighbours move by DPP lane shifts, memory
+// is addressed as a uniform base plus a 32-bit per-lane byte offset, the
+// prefetch slots are emptied through keep(), and groups of four columns in
+// the all-ones x-profile range run a fast path without profile loads.  The
+// residual injection is a small kernel of its own (k_inject), so no tile of
+// the step waits on a receiver lookup.
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "sam2d_device.h"
 #include "sam2d_kcommon.h"
 
 namespace sam2d {
 
+// Residual injection before backward step n: for each distinct receiver cell,
+// the residuals of its receivers (sample rec_k, receiver order) are summed and
+// added to L_px and L_pz of the adjoint state slot `lam` (the transpose of
+// recording p = px + pz there, cpu_record, cpu_modelling_kernels.cpp:24-31).
+__global__ void k_inject(const AdjDev *__restrict__ shots, int nshots, int lam, int rec_k)
+{
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gstride = gridDim.x * blockDim.x;
+    for (int s = 0; s < nshots; ++s) {
+        const AdjDev &d = shots[s];
+        SAM2D_GLOBAL float *Lpx = gp(d.lam + (4 * (size_t)lam + 0) * d.ncell);
+        SAM2D_GLOBAL float *Lpz = gp(d.lam + (4 * (size_t)lam + 1) * d.ncell);
+        for (int u = gtid; u < d.ncells; u += gstride) {
+            float sum = 0.0f;
+            for (int e = d.rptr[u]; e < d.rptr[u + 1]; e++)
+                sum += gp(d.res)[(size_t)d.rid[e] * d.real_nt + rec_k];
+            const int c = d.rcell[u];
+            Lpx[c] += sum;
+            Lpz[c] += sum;
+        }
+    }
+}
+
 struct AdjCol {
-    float lvx_n, bx_n, sbx_n, fvx_n;   // column c+1: L_vx, BX, damp2b1, fwd vx[n-1]
-    float lvz, bz;                     // column c:   L_vz, BZ
-    float lpx, lpz;                    //             L_px, L_pz
-    float fvz, fpx, fpz;               //             fwd vz[n-1], px[n], pz[n]
-    float kx, kz, dax, skx, gk;        //             KX, KZ, damp2a, damp2a1, dJ/dkappa
-    float lvz_j, gbx_j, gbz_j;         // column j:   L_vz, dJ/dbu2, dJ/dbu1
-    float dbx_j, sbx_j;                //             damp2b, damp2b1
+    float lvx_n, bx_n, fvx_n;   // column c+1: L_vx, BX, fwd vx[n-1]
+    float lvz, bz;              // column c:   L_vz, BZ
+    float lpx, lpz;             //             L_px, L_pz
+    float fvz, fpx, fpz;        //             fwd vz[n-1], px[n], pz[n]
+    float kx, kz, gk;           //             KX, KZ (unused when aliased), dJ/dkappa
+    float lvz_j, gbx_j, gbz_j;  // column j:   L_vz, dJ/dbu2, dJ/dbu1
 };
 
-template <int WAVES>
+template <int WAVES, int PF, bool KZA>
 __global__ __launch_bounds__(WAVES * 64)
 void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
-               int ntiles, int step, int lam_in, int lam_out, int fwd_prev, int fwd_cur,
-               int rec_k)
+               int ntiles, int step, int lam_in, int lam_out, int fwd_prev, int fwd_cur)
 {
-    constexpr int PF = 2;
+    static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
+    using gcchar = const SAM2D_GLOBAL char;
+    using gchar = SAM2D_GLOBAL char;
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
     for (int v = blockIdx.x; v < vtotal; v += gridDim.x) {
         const int tile = xcd_tile(v, per_xcd);
@@ -63,36 +94,55 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         const int chunk = local / d.ngroups;
         const int nnz = d.nnz, nnx = d.nnx;
         const int zb = (group * WAVES + wv) * OWN_Z;
-        if (zb >= nnz) continue;
+        if (zb >= nnz) continue;                      // wave-uniform
         const int xa = chunk * d.chunk;
         const int xb = min(xa + d.chunk, nnx);
 
         const int r = zb - HALO_T + lane;
         const bool rown = (lane >= HALO_T) && (lane < HALO_T + OWN_Z) && (r < nnz);
         const unsigned rc = (unsigned)min(max(r, 0), nnz - 1);
+        const unsigned rc4 = rc * 4u;
         const size_t N = d.ncell;
 
-        const SAM2D_GLOBAL float *__restrict__ Lpx = gp(d.lam + (4 * (size_t)lam_in + 0) * N);
-        const SAM2D_GLOBAL float *__restrict__ Lpz = gp(d.lam + (4 * (size_t)lam_in + 1) * N);
-        const SAM2D_GLOBAL float *__restrict__ Lvx = gp(d.lam + (4 * (size_t)lam_in + 2) * N);
-        const SAM2D_GLOBAL float *__restrict__ Lvz = gp(d.lam + (4 * (size_t)lam_in + 3) * N);
-        SAM2D_GLOBAL float *__restrict__ Opx = gp(d.lam + (4 * (size_t)lam_out + 0) * N);
-        SAM2D_GLOBAL float *__restrict__ Opz = gp(d.lam + (4 * (size_t)lam_out + 1) * N);
-        SAM2D_GLOBAL float *__restrict__ Ovx = gp(d.lam + (4 * (size_t)lam_out + 2) * N);
-        SAM2D_GLOBAL float *__restrict__ Ovz = gp(d.lam + (4 * (size_t)lam_out + 3) * N);
-        const SAM2D_GLOBAL float *__restrict__ Fpx = gp(d.fwd + (4 * (size_t)fwd_cur + 0) * N);
-        const SAM2D_GLOBAL float *__restrict__ Fpz = gp(d.fwd + (4 * (size_t)fwd_cur + 1) * N);
-        const SAM2D_GLOBAL float *__restrict__ Fvx = gp(d.fwd + (4 * (size_t)fwd_prev + 2) * N);
-        const SAM2D_GLOBAL float *__restrict__ Fvz = gp(d.fwd + (4 * (size_t)fwd_prev + 3) * N);
-        const SAM2D_GLOBAL float *__restrict__ KX = gp(d.KX);
-        const SAM2D_GLOBAL float *__restrict__ KZ = gp(d.KZ);
-        const SAM2D_GLOBAL float *__restrict__ BX = gp(d.BX);
-        const SAM2D_GLOBAL float *__restrict__ BZ = gp(d.BZ);
+        // Adjoint state, forward ring slots, coefficients and accumulators
+        // never overlap.
+        gcchar *__restrict__ Lpx = (gcchar *)(d.lam + (4 * (size_t)lam_in + 0) * N);
+        gcchar *__restrict__ Lpz = (gcchar *)(d.lam + (4 * (size_t)lam_in + 1) * N);
+        gcchar *__restrict__ Lvx = (gcchar *)(d.lam + (4 * (size_t)lam_in + 2) * N);
+        gcchar *__restrict__ Lvz = (gcchar *)(d.lam + (4 * (size_t)lam_in + 3) * N);
+        gchar *__restrict__ Opx = (gchar *)(d.lam + (4 * (size_t)lam_out + 0) * N);
+        gchar *__restrict__ Opz = (gchar *)(d.lam + (4 * (size_t)lam_out + 1) * N);
+        gchar *__restrict__ Ovx = (gchar *)(d.lam + (4 * (size_t)lam_out + 2) * N);
+        gchar *__restrict__ Ovz = (gchar *)(d.lam + (4 * (size_t)lam_out + 3) * N);
+        gcchar *__restrict__ Fpx = (gcchar *)(d.fwd + (4 * (size_t)fwd_cur + 0) * N);
+        gcchar *__restrict__ Fpz = (gcchar *)(d.fwd + (4 * (size_t)fwd_cur + 1) * N);
+        gcchar *__restrict__ Fvx = (gcchar *)(d.fwd + (4 * (size_t)fwd_prev + 2) * N);
+        gcchar *__restrict__ Fvz = (gcchar *)(d.fwd + (4 * (size_t)fwd_prev + 3) * N);
+        gcchar *__restrict__ KX = (gcchar *)d.KX;
+        gcchar *__restrict__ KZ = (gcchar *)d.KZ;
+        gcchar *__restrict__ BX = (gcchar *)d.BX;
+        gcchar *__restrict__ BZ = (gcchar *)d.BZ;
+        // The accumulators are read at a column and written back at the same
+        // cell by the same lane only.
+        gchar *__restrict__ GK = (gchar *)d.gk;
+        gchar *__restrict__ GBX = (gchar *)d.gbx;
+        gchar *__restrict__ GBZ = (gchar *)d.gbz;
         const SAM2D_GLOBAL float *__restrict__ prx = gp(d.prof_x);
         const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
-        SAM2D_GLOBAL float *__restrict__ GK = gp(d.gk);
-        SAM2D_GLOBAL float *__restrict__ GBX = gp(d.gbx);
-        SAM2D_GLOBAL float *__restrict__ GBZ = gp(d.gbz);
+
+        auto ld = [](gcchar *__restrict__ base, unsigned off) -> float {
+            return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
+        };
+        auto ldw = [](gchar *__restrict__ base, unsigned off) -> float {
+            return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
+        };
+        auto stf = [](gchar *__restrict__ base, unsigned off, float val) {
+            *(SAM2D_GLOBAL float *__restrict__)(base + off) = val;
+        };
+        auto coff = [&](int col) -> unsigned {
+            return (unsigned)min(max(col, 0), nnx - 1) * (unsigned)nnz * 4u + rc4;
+        };
+        auto clampc = [&](int col) -> int { return min(max(col, 0), nnx - 1); };
 
         const float daz = prz[rc], skz = prz[nnz + rc];
         const float dbz = prz[2 * nnz + rc], sbz = prz[3 * nnz + rc];
@@ -103,36 +153,30 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         const bool src_lane = (r == d.src_z);
         const float srcg = (step < d.nts) ? gp(d.srcg)[step] : 0.0f;
         const int src_x = (step < d.nts) ? d.src_x : -1;
-        const SAM2D_GLOBAL int *colptr = gp(d.colptr);
-        const SAM2D_GLOBAL int *rrow = gp(d.rrow);
-        const SAM2D_GLOBAL int *rid = gp(d.rid);
-        const SAM2D_GLOBAL float *res = gp(d.res);
+        // Fast groups start at c0 in [fast_lo, fast_hi]: columns c0-2 .. c0+4
+        // in the all-ones x-profile range, 4th-order stencils at c0-2 .. c0+4.
+        const int fast_lo = max(d.xin_lo + 2, 3);
+        const int fast_hi = min(d.xin_hi - 4, nnx - 7);
 
-        auto colbase = [&](int col) -> unsigned {
-            return (unsigned)min(max(col, 0), nnx - 1) * (unsigned)nnz;
-        };
         auto load_col = [&](int c) -> AdjCol {
             AdjCol o;
-            const unsigned b1 = colbase(c + 1), b0 = colbase(c), bj = colbase(c - 2);
-            const int c1 = min(max(c + 1, 0), nnx - 1), cc = min(max(c, 0), nnx - 1);
-            const int jc = min(max(c - 2, 0), nnx - 1);
-            o.lvx_n = Lvx[b1 + rc]; o.bx_n = BX[b1 + rc]; o.sbx_n = prx[3 * nnx + c1];
-            o.fvx_n = Fvx[b1 + rc];
-            o.lvz = Lvz[b0 + rc]; o.bz = BZ[b0 + rc];
-            o.lpx = Lpx[b0 + rc]; o.lpz = Lpz[b0 + rc];
-            o.fvz = Fvz[b0 + rc]; o.fpx = Fpx[b0 + rc]; o.fpz = Fpz[b0 + rc];
-            o.kx = KX[b0 + rc]; o.kz = KZ[b0 + rc];
-            o.dax = prx[cc]; o.skx = prx[nnx + cc]; o.gk = GK[b0 + rc];
-            o.lvz_j = Lvz[bj + rc]; o.gbx_j = GBX[bj + rc]; o.gbz_j = GBZ[bj + rc];
-            o.dbx_j = prx[2 * nnx + jc]; o.sbx_j = prx[3 * nnx + jc];
+            const unsigned o1 = coff(c + 1), o0 = coff(c), oj = coff(c - 2);
+            o.lvx_n = ld(Lvx, o1); o.bx_n = ld(BX, o1); o.fvx_n = ld(Fvx, o1);
+            o.lvz = ld(Lvz, o0); o.bz = ld(BZ, o0);
+            o.lpx = ld(Lpx, o0); o.lpz = ld(Lpz, o0);
+            o.fvz = ld(Fvz, o0); o.fpx = ld(Fpx, o0); o.fpz = ld(Fpz, o0);
+            o.kx = ld(KX, o0);
+            o.kz = KZA ? 0.0f : ld(KZ, o0);
+            o.gk = ldw(GK, o0);
+            o.lvz_j = ld(Lvz, oj); o.gbx_j = ldw(GBX, oj); o.gbz_j = ldw(GBZ, oj);
             return o;
         };
         // Per-column products of the transposed D+ (x): c0(i) gx(i), c1(i) gx(i)
         // with gx = BXe L_vx at column i.
-        auto gx_terms = [&](int i, float lvx, float bx, float sbx, float &a, float &b) {
+        auto gx_terms = [&](int i, float lvx, float bx, float &a, float &b) {
             float p0, p1;
             dplus_coef(i, nnx, p0, p1);
-            const float g = (bx * sbx) * lvx;
+            const float g = (bx * prx[3 * nnx + clampc(i)]) * lvx;
             a = p0 * g;
             b = p1 * g;
         };
@@ -148,112 +192,169 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int col = xa - 3 + k;
-            const unsigned b = colbase(col);
-            const int cl = min(max(col, 0), nnx - 1);
-            LVX[k] = Lvx[b + rc];
-            FVX[k] = Fvx[b + rc];
-            gx_terms(col, LVX[k], BX[b + rc], prx[3 * nnx + cl], AXP[k], BXP[k]);
+            const unsigned o = coff(col);
+            LVX[k] = ld(Lvx, o);
+            FVX[k] = ld(Fvx, o);
+            gx_terms(col, LVX[k], ld(BX, o), AXP[k], BXP[k]);
         }
 
         AdjCol q[PF];
         #pragma unroll
         for (int u = 0; u < PF; u++) q[u] = load_col(xa - 1 + u);
 
-        for (int c0 = xa - 1; c0 <= xb + 1; c0 += 4) {
-            #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int c = c0 + u;
-                AdjCol &cur = q[u % PF];
-                // ---- phase A at column c
-                LVX[(u + 3) & 3] = cur.lvx_n;
-                FVX[(u + 3) & 3] = cur.fvx_n;
-                gx_terms(c + 1, cur.lvx_n, cur.bx_n, cur.sbx_n, AXP[(u + 3) & 3], BXP[(u + 3) & 3]);
-                const float ax = AXP[(u + 1) & 3] - AXP[(u + 2) & 3]
-                               + (BXP[u & 3] - BXP[(u + 3) & 3]);
-                const float gz = (cur.bz * sbz) * cur.lvz;
-                const float az = zp0 * gz, bzv = zp1 * gz;
-                const float azz = (shfl_up(az, 1) - az) + (shfl_up(bzv, 2) - shfl_dn(bzv, 1));
-                float lp = ax + azz;
-                if (rec_k >= 0 && c >= 0 && c < nnx) {
-                    const int e1 = colptr[c + 1];
-                    for (int e = colptr[c]; e < e1; e++) {
-                        const int row = rrow[e];
-                        const float val = res[(size_t)rid[e] * d.real_nt + rec_k];
-                        if (r == row) lp += val;
-                    }
-                }
-                const float lpx1 = cur.lpx + lp;
-                const float lpz1 = cur.lpz + lp;
-                float xm0, xm1;
-                dminus_coef(c, nnx, xm0, xm1);
-                // dJ/dkappa from the p-update needs Dx- vx[n-1] and Dz- vz[n-1];
-                // the lane shuffles run with every lane active (a shuffle only
-                // sees the values of active lanes).
-                const float f0 = cur.fvz;
-                const float f_u1 = shfl_up(f0, 1), f_u2 = shfl_up(f0, 2), f_d1 = shfl_dn(f0, 1);
-                if (rown && c >= xa && c < xb) {
-                    const float dvx = fmaf(xm0, FVX[(u + 2) & 3] - FVX[(u + 1) & 3],
-                                           xm1 * (FVX[(u + 3) & 3] - FVX[u & 3]));
-                    const float dvz = fmaf(zm0, f0 - f_u1, zm1 * (f_d1 - f_u2));
-                    float g = cur.gk + lpx1 * dvx * (dtdx * cur.skx) + lpz1 * dvz * (dtdz * skz);
-                    if (src_lane && c == src_x) g += lpx1 * srcg;
-                    const unsigned id = (unsigned)c * (unsigned)nnz + (unsigned)r;
-                    GK[id] = g;
-                    Opx[id] = cur.dax * lpx1;
-                    Opz[id] = daz * lpz1;
-                }
-                const float hx = (cur.kx * cur.skx) * lpx1;
-                const float hz = (cur.kz * skz) * lpz1;
-                AM[(u + 3) & 3] = xm0 * hx;
-                BM[(u + 3) & 3] = xm1 * hx;
-                AMZ[(u + 3) & 3] = zm0 * hz;
-                BMZ[(u + 3) & 3] = zm1 * hz;
-                PN[(u + 3) & 3] = cur.fpx + cur.fpz;
+        auto column = [&](auto fast_tag, auto u_tag, int c0) {
+            constexpr bool FAST = decltype(fast_tag)::value;
+            constexpr int u = decltype(u_tag)::value;
+            constexpr int i0 = u & 3, i1 = (u + 1) & 3, i2 = (u + 2) & 3, i3 = (u + 3) & 3;
+            const int c = c0 + u;
+            AdjCol &cur = q[u % PF];
 
-                // ---- phase B at column j = c - 2
-                const int j = c - 2;
-                if (j >= xa && j < xb) {
-                    const float tx = AM[(u + 1) & 3] - AM[(u + 2) & 3]
-                                   + (BM[u & 3] - BM[(u + 3) & 3]);
-                    const float amz = AMZ[(u + 1) & 3], bmz = BMZ[(u + 1) & 3];
-                    const float tz = (amz - shfl_dn(amz, 1)) + (shfl_up(bmz, 1) - shfl_dn(bmz, 2));
-                    const float lvx_j = LVX[u & 3];
-                    const float pj = PN[(u + 1) & 3];
-                    const float q_u1 = shfl_up(pj, 1), q_d1 = shfl_dn(pj, 1), q_d2 = shfl_dn(pj, 2);
-                    if (rown) {
-                        float xp0, xp1;
-                        dplus_coef(j, nnx, xp0, xp1);
-                        const float dpx = fmaf(xp0, PN[(u + 2) & 3] - pj, xp1 * (PN[(u + 3) & 3] - PN[u & 3]));
-                        const float dpz = fmaf(zp0, q_d1 - pj, zp1 * (q_d2 - q_u1));
-                        const unsigned id = (unsigned)j * (unsigned)nnz + (unsigned)r;
-                        Ovx[id] = fmaf(cur.dbx_j, lvx_j, tx);
-                        Ovz[id] = fmaf(dbz, cur.lvz_j, tz);
-                        GBX[id] = cur.gbx_j + lvx_j * dpx * (dtdx * cur.sbx_j);
-                        GBZ[id] = cur.gbz_j + cur.lvz_j * dpz * (dtdz * sbz);
-                    }
-                }
-                cur = load_col(c + PF);
+            // ---- phase A at column c
+            const float lvx_n = keep(cur.lvx_n);
+            LVX[i3] = lvx_n;
+            FVX[i3] = keep(cur.fvx_n);
+            if (FAST) {
+                const float g = cur.bx_n * lvx_n;
+                AXP[i3] = SAM2D_A0 * g;
+                BXP[i3] = SAM2D_A1 * g;
+            } else {
+                gx_terms(c + 1, lvx_n, cur.bx_n, AXP[i3], BXP[i3]);
             }
+            const float ax = AXP[i1] - AXP[i2] + (BXP[i0] - BXP[i3]);
+            const float gz = (cur.bz * sbz) * cur.lvz;
+            const float az = zp0 * gz, bzv = zp1 * gz;
+            const float bz_u1 = lane_up1(bzv);
+            const float azz = (lane_up1(az) - az) + (lane_up1(bz_u1) - lane_dn1(bzv));
+            const float lp = ax + azz;
+            const float lpx1 = cur.lpx + lp;
+            const float lpz1 = cur.lpz + lp;
+            float xm0 = SAM2D_A0, xm1 = SAM2D_A1;
+            if (!FAST) dminus_coef(c, nnx, xm0, xm1);
+            // dJ/dkappa from the p-update needs Dx- vx[n-1] and Dz- vz[n-1];
+            // the lane shifts run with every lane active.
+            const float f0 = cur.fvz;
+            const float f_u1 = lane_up1(f0), f_u2 = lane_up1(f_u1), f_d1 = lane_dn1(f0);
+            float skx = 1.0f, dax = 1.0f;
+            if (!FAST) {
+                const int cc = clampc(c);
+                dax = prx[cc];
+                skx = prx[nnx + cc];
+            }
+            if (rown && c >= xa && c < xb) {
+                const float dvx = fmaf(xm0, FVX[i2] - FVX[i1], xm1 * (FVX[i3] - FVX[i0]));
+                const float dvz = fmaf(zm0, f0 - f_u1, zm1 * (f_d1 - f_u2));
+                float g = cur.gk + lpx1 * dvx * (dtdx * skx) + lpz1 * dvz * (dtdz * skz);
+                if (src_lane && c == src_x) g += lpx1 * srcg;
+                const unsigned id = (unsigned)c * (unsigned)nnz * 4u + rc4;
+                stf(GK, id, g);
+                stf(Opx, id, dax * lpx1);
+                stf(Opz, id, daz * lpz1);
+            }
+            const float kz = KZA ? cur.kx : cur.kz;
+            const float hx = (cur.kx * skx) * lpx1;
+            const float hz = (kz * skz) * lpz1;
+            AM[i3] = xm0 * hx;
+            BM[i3] = xm1 * hx;
+            AMZ[i3] = zm0 * hz;
+            BMZ[i3] = zm1 * hz;
+            PN[i3] = cur.fpx + cur.fpz;
+
+            // ---- phase B at column j = c - 2
+            const int j = c - 2;
+            const float tx = AM[i1] - AM[i2] + (BM[i0] - BM[i3]);
+            const float amz = AMZ[i1], bmz = BMZ[i1];
+            const float bm_d1 = lane_dn1(bmz);
+            const float tz = (amz - lane_dn1(amz)) + (lane_up1(bmz) - lane_dn1(bm_d1));
+            const float lvx_j = LVX[i0];
+            const float pj = PN[i1];
+            const float q_u1 = lane_up1(pj), q_d1 = lane_dn1(pj), q_d2 = lane_dn1(q_d1);
+            if (rown && j >= xa && j < xb) {
+                float xp0 = SAM2D_A0, xp1 = SAM2D_A1, dbx = 1.0f, sbx = 1.0f;
+                if (!FAST) {
+                    dplus_coef(j, nnx, xp0, xp1);
+                    const int jc = clampc(j);
+                    dbx = prx[2 * nnx + jc];
+                    sbx = prx[3 * nnx + jc];
+                }
+                const float dpx = fmaf(xp0, PN[i2] - pj, xp1 * (PN[i3] - PN[i0]));
+                const float dpz = fmaf(zp0, q_d1 - pj, zp1 * (q_d2 - q_u1));
+                const unsigned id = (unsigned)j * (unsigned)nnz * 4u + rc4;
+                stf(Ovx, id, fmaf(dbx, lvx_j, tx));
+                stf(Ovz, id, fmaf(dbz, cur.lvz_j, tz));
+                stf(GBX, id, cur.gbx_j + lvx_j * dpx * (dtdx * sbx));
+                stf(GBZ, id, cur.gbz_j + cur.lvz_j * dpz * (dtdz * sbz));
+            }
+            cur = load_col(c + PF);
+        };
+
+        using TT = std::integral_constant<bool, true>;
+        using FF = std::integral_constant<bool, false>;
+        using U0 = std::integral_constant<int, 0>;
+        using U1 = std::integral_constant<int, 1>;
+        using U2 = std::integral_constant<int, 2>;
+        using U3 = std::integral_constant<int, 3>;
+        // Slow groups, the fast range, slow groups.
+        const int c_last = xb + 1;
+        int c0 = xa - 1;
+        for (; c0 <= c_last && c0 < fast_lo; c0 += 4) {
+            column(FF(), U0(), c0); column(FF(), U1(), c0);
+            column(FF(), U2(), c0); column(FF(), U3(), c0);
+        }
+        const int c_fast = min(fast_hi, c_last);
+        for (; c0 <= c_fast; c0 += 4) {
+            column(TT(), U0(), c0); column(TT(), U1(), c0);
+            column(TT(), U2(), c0); column(TT(), U3(), c0);
+        }
+        for (; c0 <= c_last; c0 += 4) {
+            column(FF(), U0(), c0); column(FF(), U1(), c0);
+            column(FF(), U2(), c0); column(FF(), U3(), c0);
         }
     }
 }
 
-hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
-                               int step, int lam_in, int lam_out, int fwd_prev, int fwd_cur,
-                               int rec_k, hipStream_t stream)
+template <int W, int PF, bool KZA>
+static void launch_adj_t(dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt, int per, int nt,
+                         int step, int li, int lo, int fp, int fc)
 {
-    int waves, chunk;
-    variant_shape(variant, &waves, &chunk);
+    hipLaunchKernelGGL((k_adjoint<W, PF, KZA>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step, li,
+                       lo, fp, fc);
+}
+
+template <int W>
+static void launch_adj_w(int pf, bool kza, dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt,
+                         int per, int nt, int step, int li, int lo, int fp, int fc)
+{
+    if (pf == 2 && kza) launch_adj_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc);
+    else if (pf == 2) launch_adj_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc);
+    else if (kza) launch_adj_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc);
+    else launch_adj_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc);
+}
+
+hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
+                               bool kz_alias, int step, int lam_in, int lam_out, int fwd_prev,
+                               int fwd_cur, hipStream_t stream)
+{
+    int waves, chunk, pf;
+    variant_shape_pf(variant, &waves, &chunk, &pf);
     const int per = (ntiles + 7) / 8;
     const int vtotal = per * 8;
     const dim3 g(vtotal);
     switch (waves) {
-    case 1: hipLaunchKernelGGL(k_adjoint<1>, g, dim3(64), 0, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
-    case 2: hipLaunchKernelGGL(k_adjoint<2>, g, dim3(128), 0, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
-    case 4: hipLaunchKernelGGL(k_adjoint<4>, g, dim3(256), 0, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
-    case 8: hipLaunchKernelGGL(k_adjoint<8>, g, dim3(512), 0, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
+    case 1: launch_adj_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur); break;
+    case 2: launch_adj_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur); break;
+    case 4: launch_adj_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur); break;
+    case 8: launch_adj_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_inject(const AdjDev *d_adj, int nshots, int max_cells, int lam, int rec_k,
+                         hipStream_t stream)
+{
+    if (max_cells <= 0 || rec_k < 0) return hipSuccess;
+    const int blocks = (max_cells + 255) / 256;
+    hipLaunchKernelGGL(k_inject, dim3(blocks), dim3(256), 0, stream, d_adj, nshots, lam, rec_k);
     return hipGetLastError();
 }
 

@@ -53,6 +53,8 @@ struct AdjDev {
     int nnz, nnx;
     int ngroups, nchunks, chunk, tile_begin;
     int src_x, src_z, nts, real_nt;
+    int xin_lo, xin_hi;    // columns where all four x-profiles are exactly 1
+    int ncells, pad0;      // distinct receiver cells
     float dtdx, dtdz;      // fd_dt/dx, fd_dt/dz as the coefficients use them
     const float *srcg;     // [nts] d(source increment)/d(kappa_s) = fd_dt*w[n]/(dx*dz)
     const float *KX, *KZ, *BX, *BZ;
@@ -63,18 +65,21 @@ struct AdjDev {
     float *gk;             // dJ/dkappa      (padded grid)
     float *gbx;            // dJ/dbu2        (x-staggered buoyancy)
     float *gbz;            // dJ/dbu1        (z-staggered buoyancy)
-    const int *colptr;     // [nnx+1] receivers of each column (CSR)
-    const int *rrow;       // [ng] padded row of each entry
-    const int *rid;        // [ng] receiver index of each entry
+    const int *rcell;      // [ncells] distinct receiver cells (ix*nnz + iz)
+    const int *rptr;       // [ncells+1] receivers of each cell (CSR)
+    const int *rid;        // [ng] receiver index of each entry, ascending per cell
     const float *res;      // [ng][real_nt] adjoint source dJ/dseis
 };
 
 // One backward step n of every staged shot: adjoint state lam_in -> lam_out,
-// forward states S(n-1) in ring slot fwd_prev and S(n) in fwd_cur; rec_k >= 0
-// injects residual sample rec_k.
+// forward states S(n-1) in ring slot fwd_prev and S(n) in fwd_cur.
+// kz_alias: every staged shot's KZ is its KX.
 hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
-                               int step, int lam_in, int lam_out, int fwd_prev, int fwd_cur,
-                               int rec_k, hipStream_t stream);
+                               bool kz_alias, int step, int lam_in, int lam_out, int fwd_prev,
+                               int fwd_cur, hipStream_t stream);
+// Residual sample rec_k into adjoint slot lam before a backward step.
+hipError_t launch_inject(const AdjDev *d_adj, int nshots, int max_cells, int lam, int rec_k,
+                         hipStream_t stream);
 
 int num_variants();
 void variant_shape(int v, int *waves, int *chunk);
@@ -99,6 +104,11 @@ hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int nti
                         bool kz_alias, int step, int in_slot, int out_slot, int band_mask,
                         int band_w, const int *band_samples, int state_sample, int max_ng,
                         hipStream_t stream);
+// The recomputation form: the states after all three steps of a three-step
+// variant are stored, into slots in_slot+1 .. in_slot+3 (no receivers).
+bool ring_supported(int variant);
+hipError_t launch_stepn_ring(int variant, const ShotDev *d_shots, int nshots, int ntiles,
+                             int grid_cap, bool kz_alias, int step, int in_slot, hipStream_t stream);
 // The deferred gathers as a launch of their own (after the last multi-step
 // launch): band_samples[j] from slice j of parity band_r, the state slot.
 hipError_t launch_gather(const ShotDev *d_shots, int nshots, int max_ng, int band_r,

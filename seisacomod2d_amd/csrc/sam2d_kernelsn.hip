@@ -46,7 +46,11 @@ struct ColN {
     float bx, bz, vz_j;         // BX, BZ, vz(n-1) at column c-2
 };
 
-template <int T, int WAVES, int PF, bool KZA>
+// RING: the states after the intermediate steps are stored as well, into
+// state slots in_slot+1 .. in_slot+T-1 (out_slot = in_slot+T): the
+// recomputation of a checkpoint segment for the adjoint, which needs every
+// state, at the multi-step kernel's read traffic.
+template <int T, int WAVES, int PF, bool KZA, bool RING>
 __global__ __launch_bounds__(WAVES * 64)
 void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_xcd, int ntiles,
              int step, int in_slot, int out_slot, int band_mask, int band_w, int4 band_s,
@@ -99,6 +103,14 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
         gcchar *__restrict__ BZ = (gcchar *)d.BZ;
         const SAM2D_GLOBAL float *__restrict__ prx = gp(d.prof_x);
         const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
+        // Intermediate-state slots (RING only).
+        gchar *__restrict__ mid[T][4];
+        #pragma unroll
+        for (int t = 0; t < T; t++)
+            #pragma unroll
+            for (int f = 0; f < 4; f++)
+                mid[t][f] = RING ? (gchar *)slot_field(d, in_slot + 1 + (t < T - 1 ? t : 0), f)
+                                 : nullptr;
 
         auto ld = [](gcchar *__restrict__ base, unsigned off) -> float {
             return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
@@ -229,6 +241,11 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                         if (((band_mask >> t) & 1) && strip_band && cp >= xa && cp < xb) {
                             if (band_lane) bandp[t][(size_t)r * nnx + cp] = P[t][i3];
                         }
+                        if (RING && rown && cp >= xa && cp < xb) {
+                            const unsigned id = (unsigned)cp * (unsigned)nnz * 4u + rc4;
+                            stf(mid[t][0], id, pxn);
+                            stf(mid[t][1], id, pzn);
+                        }
                     } else if (rown && cp >= xa && cp < xb) {
                         const unsigned id = (unsigned)cp * (unsigned)nnz * 4u + rc4;
                         stf(pxo, id, pxn);
@@ -267,6 +284,11 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                         VZ[t][i3] = fmaf(bz * sbz, dpz, dbz * vz_old);
                         BXw[t][i3] = bx;
                         BZw[t][i3] = bz;
+                        if (RING && rown && jv >= xa && jv < xb) {
+                            const unsigned id = (unsigned)jv * (unsigned)nnz * 4u + rc4;
+                            stf(mid[t][2], id, VX[t][i3]);
+                            stf(mid[t][3], id, VZ[t][i3]);
+                        }
                     } else if (rown && jv >= xa && jv < xb) {
                         float vxn;
                         if (FAST) {
@@ -323,31 +345,31 @@ struct StepNArgs {
     int ss;
 };
 
-template <int T, int W, int PF, bool KZA>
+template <int T, int W, int PF, bool KZA, bool RING>
 static void launchn_t(dim3 g, hipStream_t s, const StepNArgs &a)
 {
-    hipLaunchKernelGGL((k_stepn<T, W, PF, KZA>), g, dim3(64 * W), 0, s, a.d, a.ns, a.vt, a.per,
-                       a.nt, a.step, a.in, a.out, a.bmask, a.bw, a.bs, a.ss);
+    hipLaunchKernelGGL((k_stepn<T, W, PF, KZA, RING>), g, dim3(64 * W), 0, s, a.d, a.ns, a.vt,
+                       a.per, a.nt, a.step, a.in, a.out, a.bmask, a.bw, a.bs, a.ss);
 }
 
-template <int T, int W>
+template <int T, int W, bool RING>
 static void launchn_w(int pf, bool kza, dim3 g, hipStream_t s, const StepNArgs &a)
 {
-    if (pf == 2 && kza) launchn_t<T, W, 2, true>(g, s, a);
-    else if (pf == 2) launchn_t<T, W, 2, false>(g, s, a);
-    else if (kza) launchn_t<T, W, 4, true>(g, s, a);
-    else launchn_t<T, W, 4, false>(g, s, a);
+    if (pf == 2 && kza) launchn_t<T, W, 2, true, RING>(g, s, a);
+    else if (pf == 2) launchn_t<T, W, 2, false, RING>(g, s, a);
+    else if (kza) launchn_t<T, W, 4, true, RING>(g, s, a);
+    else launchn_t<T, W, 4, false, RING>(g, s, a);
 }
 
-template <int T>
+template <int T, bool RING = false>
 static hipError_t launchn_waves(int waves, int pf, bool kza, dim3 g, hipStream_t s,
                                 const StepNArgs &a)
 {
     switch (waves) {
-    case 1: launchn_w<T, 1>(pf, kza, g, s, a); break;
-    case 2: launchn_w<T, 2>(pf, kza, g, s, a); break;
-    case 4: launchn_w<T, 4>(pf, kza, g, s, a); break;
-    case 8: launchn_w<T, 8>(pf, kza, g, s, a); break;
+    case 1: launchn_w<T, 1, RING>(pf, kza, g, s, a); break;
+    case 2: launchn_w<T, 2, RING>(pf, kza, g, s, a); break;
+    case 4: launchn_w<T, 4, RING>(pf, kza, g, s, a); break;
+    case 8: launchn_w<T, 8, RING>(pf, kza, g, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipSuccess;
@@ -383,6 +405,25 @@ hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int nti
     case 5: e = launchn_waves<5>(waves, pf, kz_alias, g, stream, a); break;
     default: return hipErrorInvalidValue;
     }
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+bool ring_supported(int variant) { return variant_steps(variant) == 3; }
+
+hipError_t launch_stepn_ring(int variant, const ShotDev *d_shots, int nshots, int ntiles,
+                             int grid_cap, bool kz_alias, int step, int in_slot, hipStream_t stream)
+{
+    if (!ring_supported(variant)) return hipErrorInvalidValue;
+    int waves, chunk, pf;
+    variant_shape_pf(variant, &waves, &chunk, &pf);
+    const int per = (ntiles + 7) / 8;
+    const int vtotal = per * 8;
+    int grid = vtotal;
+    if (grid_cap > 0 && grid > grid_cap) grid = (grid_cap + 7) / 8 * 8;
+    const StepNArgs a{d_shots, nshots, vtotal, per, ntiles, step, in_slot, in_slot + 3,
+                      0, 0, make_int4(-1, -1, -1, -1), -1};
+    const hipError_t e = launchn_waves<3, true>(waves, pf, kz_alias, dim3(grid), stream, a);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
