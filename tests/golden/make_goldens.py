@@ -22,7 +22,6 @@ one output file and carry state between shots (cpumodelling.cpp:343-374).
 """
 import hashlib
 import os
-import subprocess
 import sys
 import tempfile
 
@@ -30,47 +29,17 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
-EXE = os.path.join(ROOT, "oracle", "_ref", "SeisAcoMod2D_MULTI_CORE_CPU_V1.0.exe")
 
-CARD = """"Model" : "comment"
-    "Velocity" : "vp.bin"
-    "Density" : "rho.bin"
-    "NX" : "{nx}"
-    "NZ" : "{nz}"
-    "DX" : "{dx}"
-    "DZ" : "{dz}"
-    "Time" : "{T}"
-    "FD DT" : "{fd_dt}"
-    "DATA DT" : "{real_dt}"
-    "Frequency" : "{fm}"
-    "Read source receiver flag" : "{flag}"
-    "if choice=0, Geometry file" : "geom.txt"
-    "if choice=1, Shot file" : "shots.txt"
-    "Output Directory" : "out"
-    "Free surface" : "0"
-    "Job type New or Restart" : "New"
-    "Order of FD Operator" : "4"
-"""
+sys.path.insert(0, ROOT)
+from oracle import refrun  # noqa: E402
 
 
 def run_reference(nx, nz, dx, dz, T, fd_dt, real_dt, fm, vp, rho, shot_line=None, geom_lines=None,
                   threads=8):
     with tempfile.TemporaryDirectory() as d:
-        vp.astype(np.float32).tofile(os.path.join(d, "vp.bin"))
-        rho.astype(np.float32).tofile(os.path.join(d, "rho.bin"))
-        flag = 1 if shot_line else 0
-        if shot_line:
-            open(os.path.join(d, "shots.txt"), "w").write(shot_line + "\n")
-        else:
-            open(os.path.join(d, "geom.txt"), "w").write("\n".join(geom_lines) + "\n")
-        open(os.path.join(d, "card.json"), "w").write(CARD.format(
-            nx=nx, nz=nz, dx=dx, dz=dz, T=T, fd_dt=fd_dt, real_dt=real_dt, fm=fm, flag=flag))
-        os.makedirs(os.path.join(d, "out"))
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-        r = subprocess.run([EXE, "card.json"], cwd=d, env=env, capture_output=True, text=True,
-                           timeout=3600)
-        if r.returncode != 0:
-            raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+        flag = refrun.write_case(d, nx, nz, dx, dz, vp, rho, shot_line=shot_line,
+                                 geom_lines=geom_lines)
+        _, r = refrun.run_card(d, nx, nz, dx, dz, T, fd_dt, real_dt, fm, flag, threads)
         seis = np.fromfile(os.path.join(d, "out", "card_seismogram.bin"), np.float32)
         geo = np.loadtxt(os.path.join(d, "test_geometry.txt"), dtype=np.float64, ndmin=2)
         log = r.stdout
@@ -125,8 +94,8 @@ def save(name, **kw):
 
 
 def main(with_c0=True):
-    if not os.path.exists(EXE):
-        sys.exit(f"{EXE} missing: make -C oracle ref")
+    if not refrun.available():
+        sys.exit(f"{refrun.EXE} missing: make -C oracle ref")
 
     # 1. two-layer, split spread from a shot file, dt_factor 1
     nx, nz, h = 100, 80, 10.0
