@@ -330,7 +330,7 @@ def anomaly_model(nx=1201, nz=601, h=10.0, seed=3):
     return true.astype(np.float32), bg.astype(np.float32), rho
 
 
-def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False):
+def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False, adj_steps=2):
     """Gradient time against the forward modelling of the same shots: the
     batched forward (autotuned, the fastest forward path) and the per-shot
     loop."""
@@ -342,6 +342,7 @@ def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False):
     with sam.Modeller(grid, v0, rho) as m:
         if no_graph:
             m.set_option("graph", 0)
+        m.set_option("adjoint_steps", adj_steps)
         m.autotune(windows=wins, nshots=len(sh), steps=6)
         d = m.forward_batched(sh, wins)                          # warm
         t0 = time.perf_counter()
@@ -356,7 +357,7 @@ def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False):
             t0 = time.perf_counter()
             gvp, grho = m.gradient(sh, adj, ckpt_k=k, windows=wins)
             t_g = time.perf_counter() - t0
-            res.append({"workload": label, "ckpt_k": k, "shots": len(sh),
+            res.append({"workload": label, "ckpt_k": k, "shots": len(sh), "adjoint_steps": adj_steps,
                         "forward_batched_s_per_shot": round(t_fwd / len(sh), 4),
                         "forward_per_shot_loop_s_per_shot": round(t_plain / len(sh), 4),
                         "gradient_s_per_shot": round(t_g / len(sh), 4),
@@ -378,7 +379,7 @@ def bench_gradient(args, sam):
     rec = np.stack([np.arange(1201) * 10.0, np.full(1201, 10.0)], 1).astype(np.float32)
     sh = [sam.Shot(10.0 * int((k + 0.5) * 1201 / n3), 10.0, rec) for k in range(n3)]
     out += _grad_case(sam, grid, vtrue, v0, rho, sh, None, [100], "config3_gradient_1201x601_nt3000",
-                      args.no_graph)
+                      args.no_graph, args.adjoint_steps)
     # config 4: salt model, smoothed starting model, windowed shots
     vs, rs = salt_model()
     from scipy.ndimage import gaussian_filter
@@ -387,7 +388,7 @@ def bench_gradient(args, sam):
     grid4 = sam.Grid(3201, 1201, 7.62, 7.62, 4.0, 0.0005, 0.0005, 15.0, npml=60)
     sh4, w4 = salt_shots(sam, n4, seed=11)
     out += _grad_case(sam, grid4, vs, v4, rs, sh4, w4, [50, 100, 200] if not args.quick else [200],
-                      "config4_gradient_3201x1201_nt8000_windows", args.no_graph)
+                      "config4_gradient_3201x1201_nt8000_windows", args.no_graph, args.adjoint_steps)
     print(json.dumps({"metric": METRIC,
                       "unit": "s/shot", "higher_is_better": False, "dtype": "f32",
                       "results": out}), flush=True)
@@ -401,6 +402,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=2, choices=(1, 2),
+                    help="gradient workload: backward steps per adjoint pass")
     ap.add_argument("--workload", default="plain", choices=["plain", "batched", "gradient"])
     ap.add_argument("--quick", action="store_true", help="batched/gradient: fewer shots")
     ap.add_argument("--max-batch", type=int, default=0)

@@ -150,7 +150,9 @@ int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
  * cells between shots, cpumodelling.cpp:343-348; default 0 = fresh state),
  * "max_batch" (shots resident at once in batched and gradient calls, 0 = all
  * or, for gradients, as many as fit "hbm_budget_gb" (default 64 GiB of device
- * memory for ring, checkpoints and adjoint state)). */
+ * memory for ring, checkpoints and adjoint state)), "adjoint_steps" (backward
+ * steps per adjoint kernel pass in sam2d_gradient, 1 or 2, default 2; both
+ * give the same bits). */
 int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value);
 
 /* Project-written autotuner: times every kernel variant (z-strips per

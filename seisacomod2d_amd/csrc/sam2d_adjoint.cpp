@@ -26,11 +26,14 @@ struct AdjShotBuffers {
     float *srcg = nullptr;
     int *rcell = nullptr, *rptr = nullptr, *rid = nullptr;
     float *res = nullptr;
-    size_t cap[7] = {0, 0, 0, 0, 0, 0, 0};   // bytes of each buffer above
+    float *rband = nullptr;
+    int *bmap = nullptr;
+    size_t cap[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // bytes of each buffer above
     ~AdjShotBuffers()
     {
         dev_free(lam); dev_free(ckpt); dev_free(srcg);
-        dev_free(rcell); dev_free(rptr); dev_free(rid); dev_free(res);
+        dev_free(rcell); dev_free(rptr); dev_free(rid); dev_free(res); dev_free(rband);
+        dev_free(bmap);
     }
 };
 
@@ -38,9 +41,11 @@ struct AdjointState {
     std::vector<std::unique_ptr<AdjShotBuffers>> bufs;
     AdjDev *d_desc = nullptr;
     int cap_desc = 0;
-    ShotDev *d_fwd = nullptr;      // forward-sweep descriptors (two-step tile shapes)
+    AdjDev *d_desc2 = nullptr;     // tile shapes of the two-step adjoint
+    int cap_desc2 = 0;
+    ShotDev *d_fwd = nullptr;      // forward-sweep descriptors (multi-step tile shapes)
     int cap_fwd = 0;
-    ~AdjointState() { dev_free(d_desc); dev_free(d_fwd); }
+    ~AdjointState() { dev_free(d_desc); dev_free(d_desc2); dev_free(d_fwd); }
 };
 
 static int grow(void **p, size_t &cap, size_t need)
@@ -126,6 +131,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     if ((int)A.bufs.size() < ns) A.bufs.resize(ns);
     std::vector<AdjDev> desc(ns);
     std::vector<int> ncells(ns, 0);
+    bool all_band = true;                // every shot's residuals come from a band
     const int nseg = (ta.fd_nt + k - 1) / k;
     for (int s = 0; s < ns; s++) {
         ShotPlan &p = plans[s];
@@ -172,6 +178,33 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         SAM2D_TRY(upload(b.rptr, rptr.data(), rptr.size() * 4, ctx->stream));
         SAM2D_TRY(upload(b.rid, rid.data(), (size_t)p.ng * 4, ctx->stream));
         SAM2D_TRY(upload(b.res, adj[s], (size_t)p.ng * ta.real_nt * 4, ctx->stream));
+        // Dense residual band over the receiver rows, [real_nt][band_h][nnx]:
+        // each receiver cell holds the sum of its receivers' residuals (in the
+        // order above), every other cell -0, so adding it is exact elsewhere.
+        int rlo = 1, rhi = 0;
+        for (int g = 0; g < p.ng; g++) {
+            const int row = p.gcell[g] % h.nnz;
+            if (g == 0 || row < rlo) rlo = row;
+            if (g == 0 || row > rhi) rhi = row;
+        }
+        const int band_h = p.ng > 0 ? rhi - rlo + 1 : 0;
+        const size_t band_n = (size_t)ta.real_nt * band_h * h.nnx;
+        bool use_band = p.ng > 0 && band_n * sizeof(float) <= ((size_t)1 << 30);
+        if (use_band) {
+            // Built on the device from the residuals: a map from band cell to
+            // receiver cell (or -1), then one pass over the band.
+            const int band_cells = band_h * h.nnx;
+            std::vector<int> bmap(band_cells, -1);
+            for (size_t u = 0; u < rcell.size(); u++)
+                bmap[(rcell[u] % h.nnz - rlo) * h.nnx + rcell[u] / h.nnz] = (int)u;
+            SAM2D_TRY(grow((void **)&b.bmap, b.cap[8], (size_t)band_cells * sizeof(int)));
+            SAM2D_TRY(grow((void **)&b.rband, b.cap[7], band_n * sizeof(float)));
+            SAM2D_TRY(upload(b.bmap, bmap.data(), (size_t)band_cells * sizeof(int), ctx->stream));
+            HIP_TRY(launch_band(b.bmap, b.rptr, b.rid, b.res, ta.real_nt, band_cells, b.rband,
+                                ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));   // the host map goes out of scope
+        }
+        all_band = all_band && (use_band || p.ng == 0);
 
         const ShotDev &fd = ctx->staged_desc[s];
         AdjDev &d = desc[s];
@@ -188,6 +221,8 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         d.xin_lo = fd.xin_lo; d.xin_hi = fd.xin_hi;
         d.ncells = ncells[s];
         d.rcell = b.rcell; d.rptr = b.rptr; d.rid = b.rid; d.res = b.res;
+        d.band_h = band_h; d.rrow_lo = rlo; d.rrow_hi = rhi;
+        d.rband = use_band ? b.rband : nullptr;
     }
     if (ns > A.cap_desc) {
         dev_free(A.d_desc); A.d_desc = nullptr; A.cap_desc = 0;
@@ -195,6 +230,27 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         A.cap_desc = ns;
     }
     SAM2D_TRY(upload(A.d_desc, desc.data(), sizeof(AdjDev) * ns, ctx->stream));
+    // Two backward steps per pass need every residual in a band (nothing to
+    // inject between the steps) and strips of OWN_Z2 rows.
+    const bool two_step = ctx->adjoint_steps == 2 && all_band;
+    int ntiles2 = 0;
+    if (two_step) {
+        int w2, c2;
+        variant_shape(variant, &w2, &c2);
+        std::vector<AdjDev> desc2(desc);
+        for (auto &dd : desc2) {
+            tile_shape(dd.nnz, dd.nnx, w2, c2, OWN_Z2, dd.ngroups, dd.nchunks);
+            dd.chunk = c2;
+            dd.tile_begin = ntiles2;
+            ntiles2 += dd.ngroups * dd.nchunks;
+        }
+        if (ns > A.cap_desc2) {
+            dev_free(A.d_desc2); A.d_desc2 = nullptr; A.cap_desc2 = 0;
+            SAM2D_TRY(dev_alloc((void **)&A.d_desc2, sizeof(AdjDev) * ns));
+            A.cap_desc2 = ns;
+        }
+        SAM2D_TRY(upload(A.d_desc2, desc2.data(), sizeof(AdjDev) * ns, ctx->stream));
+    }
     int max_ng = 0, max_cells = 0;
     for (auto &p : plans) max_ng = std::max(max_ng, p.ng);
     for (int c : ncells) max_cells = std::max(max_cells, c);
@@ -268,10 +324,20 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         for (; n < n1; n++)
             HIP_TRY(launch_step(variant, ctx->d_desc, ns, ntiles, ctx->grid_cap, n, n - n0,
                                 n - n0 + 1, -1, ctx->stream));
-        for (int n = n1 - 1; n >= n0; n--) {
-            HIP_TRY(launch_inject(A.d_desc, ns, max_cells, lam, rec_sample(ta, n), ctx->stream));
+        int n_top = n1 - 1;
+        if (two_step)
+            for (; n_top - 1 >= n0; n_top -= 2) {
+                HIP_TRY(launch_adjoint_step2(variant, A.d_desc2, ns, ntiles2, kza, n_top, lam,
+                                             lam ^ 1, n_top - n0 - 1, n_top - n0, n_top - n0 + 1,
+                                             rec_sample(ta, n_top), rec_sample(ta, n_top - 1),
+                                             ctx->stream));
+                lam ^= 1;
+            }
+        for (int n = n_top; n >= n0; n--) {
+            if (!all_band)
+                HIP_TRY(launch_inject(A.d_desc, ns, max_cells, lam, rec_sample(ta, n), ctx->stream));
             HIP_TRY(launch_adjoint_step(variant, A.d_desc, ns, ntiles, kza, n, lam, lam ^ 1, n - n0,
-                                        n - n0 + 1, ctx->stream));
+                                        n - n0 + 1, rec_sample(ta, n), ctx->stream));
             lam ^= 1;
         }
     }

@@ -8,7 +8,7 @@
 // after step n (L_px, L_pz, L_vx, L_vz) and r_n the residual injected where
 // step n was recorded:
 //
-//   L_px += r_n, L_pz += r_n at the receiver cells               (record, k_inject)
+//   L_px += r_n, L_pz += r_n at the receiver cells               (record)
 //   L_p'  = Dx+^T (BXe L_vx) + Dz+^T (BZe L_vz)                   (v-update)
 //   L_px' = L_px + L_p',   L_pz' = L_pz + L_p'                   (p = px + pz)
 //   out:  L_px = dax L_px',  L_pz = daz L_pz'                     (p sponge)
@@ -31,17 +31,24 @@
 // is addressed as a uniform base plus a 32-bit per-lane byte offset, the
 // prefetch slots are emptied through keep(), and groups of four columns in
 // the all-ones x-profile range run a fast path without profile loads.  The
-// residual injection is a small kernel of its own (k_inject), so no tile of
-// the step waits on a receiver lookup.
+// residuals come from a dense band over the receiver rows, prefetched with the
+// other operands of a column, so no tile waits on a receiver lookup; shots
+// whose band would be too large are injected by k_inject beforehand.
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <type_traits>
 #include "sam2d_device.h"
 #include "sam2d_kcommon.h"
 
 namespace sam2d {
 
-// Residual injection before backward step n: for each distinct receiver cell,
+// Every product and sum is rounded as written, so the one-step and two-step
+// adjoints, which form the same terms in different places, give the same bits.
+#pragma clang fp contract(off)
+
+// Residual injection before backward step n, for shots without a residual
+// band: for each distinct receiver cell,
 // the residuals of its receivers (sample rec_k, receiver order) are summed and
 // added to L_px and L_pz of the adjoint state slot `lam` (the transpose of
 // recording p = px + pz there, cpu_record, cpu_modelling_kernels.cpp:24-31).
@@ -51,6 +58,7 @@ __global__ void k_inject(const AdjDev *__restrict__ shots, int nshots, int lam, 
     const int gstride = gridDim.x * blockDim.x;
     for (int s = 0; s < nshots; ++s) {
         const AdjDev &d = shots[s];
+        if (d.rband) continue;                        // injected inside k_adjoint
         SAM2D_GLOBAL float *Lpx = gp(d.lam + (4 * (size_t)lam + 0) * d.ncell);
         SAM2D_GLOBAL float *Lpz = gp(d.lam + (4 * (size_t)lam + 1) * d.ncell);
         for (int u = gtid; u < d.ncells; u += gstride) {
@@ -64,19 +72,51 @@ __global__ void k_inject(const AdjDev *__restrict__ shots, int nshots, int lam, 
     }
 }
 
+// One element of the residual band per thread, consecutive threads along a
+// band row (coalesced stores).
+__global__ void k_band(const int *__restrict__ bmap, const int *__restrict__ rptr,
+                       const int *__restrict__ rid, const float *__restrict__ res, int real_nt,
+                       int band_cells, float *__restrict__ band)
+{
+    const size_t total = (size_t)real_nt * band_cells;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int k = (int)(i / band_cells);
+        const int u = bmap[i - (size_t)k * band_cells];
+        float v = -0.0f;
+        if (u >= 0)
+            for (int e = rptr[u]; e < rptr[u + 1]; e++)
+                v += res[(size_t)rid[e] * real_nt + k];
+        band[i] = v;
+    }
+}
+
+hipError_t launch_band(const int *bmap, const int *rptr, const int *rid, const float *res,
+                       int real_nt, int band_cells, float *band, hipStream_t stream)
+{
+    const size_t total = (size_t)real_nt * band_cells;
+    if (total == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((total + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_band, dim3((unsigned)blocks), dim3(256), 0, stream, bmap, rptr, rid, res,
+                       real_nt, band_cells, band);
+    return hipGetLastError();
+}
+
 struct AdjCol {
     float lvx_n, bx_n, fvx_n;   // column c+1: L_vx, BX, fwd vx[n-1]
     float lvz, bz;              // column c:   L_vz, BZ
     float lpx, lpz;             //             L_px, L_pz
     float fvz, fpx, fpz;        //             fwd vz[n-1], px[n], pz[n]
     float kx, kz, gk;           //             KX, KZ (unused when aliased), dJ/dkappa
+    float res;                  //             residual of this step (band rows)
     float lvz_j, gbx_j, gbz_j;  // column j:   L_vz, dJ/dbu2, dJ/dbu1
 };
 
 template <int WAVES, int PF, bool KZA>
 __global__ __launch_bounds__(WAVES * 64)
 void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
-               int ntiles, int step, int lam_in, int lam_out, int fwd_prev, int fwd_cur)
+               int ntiles, int step, int lam_in, int lam_out, int fwd_prev, int fwd_cur,
+               int rec_k)
 {
     static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
     using gcchar = const SAM2D_GLOBAL char;
@@ -157,6 +197,15 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         // in the all-ones x-profile range, 4th-order stencils at c0-2 .. c0+4.
         const int fast_lo = max(d.xin_lo + 2, 3);
         const int fast_hi = min(d.xin_hi - 4, nnx - 7);
+        // Residual of this step at the receiver rows, prefetched with the
+        // column (x + -0 = x, so cells without a receiver are unchanged).  Halo
+        // rows take it too: phase B of an own row reads phase A of its halo.
+        const bool tile_band = rec_k >= 0 && d.rband != nullptr &&
+                               d.rrow_lo <= zb - HALO_T + 63 && d.rrow_hi >= zb - HALO_T;
+        const bool band_lane = r >= d.rrow_lo && r <= d.rrow_hi;
+        const SAM2D_GLOBAL float *__restrict__ rb =
+            tile_band ? gp(d.rband) + ((ptrdiff_t)rec_k * d.band_h - d.rrow_lo) * (ptrdiff_t)nnx
+                      : nullptr;
 
         auto load_col = [&](int c) -> AdjCol {
             AdjCol o;
@@ -169,6 +218,10 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             o.kz = KZA ? 0.0f : ld(KZ, o0);
             o.gk = ldw(GK, o0);
             o.lvz_j = ld(Lvz, oj); o.gbx_j = ldw(GBX, oj); o.gbz_j = ldw(GBZ, oj);
+            o.res = -0.0f;
+            if (tile_band) {
+                if (band_lane) o.res = rb[(size_t)r * nnx + clampc(c)];
+            }
             return o;
         };
         // Per-column products of the transposed D+ (x): c0(i) gx(i), c1(i) gx(i)
@@ -226,8 +279,8 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             const float bz_u1 = lane_up1(bzv);
             const float azz = (lane_up1(az) - az) + (lane_up1(bz_u1) - lane_dn1(bzv));
             const float lp = ax + azz;
-            const float lpx1 = cur.lpx + lp;
-            const float lpz1 = cur.lpz + lp;
+            const float lpx1 = (cur.lpx + cur.res) + lp;
+            const float lpz1 = (cur.lpz + cur.res) + lp;
             float xm0 = SAM2D_A0, xm1 = SAM2D_A1;
             if (!FAST) dminus_coef(c, nnx, xm0, xm1);
             // dJ/dkappa from the p-update needs Dx- vx[n-1] and Dz- vz[n-1];
@@ -312,27 +365,408 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
     }
 }
 
+// Two backward steps, n and n-1, per pass: the adjoint state after step n's
+// adjoint never leaves the registers, and dJ/dkappa, dJ/dbu2, dJ/dbu1 are
+// read and written once for both steps.  A wavefront owns OWN_Z2 rows with a
+// HALO_T2-row halo on each side (the cone of two adjoint steps is that of two
+// forward steps), and at column c runs
+//   A1  step n   phase A at c        B1  step n   phase B at c-2
+//   A2  step n-1 phase A at c-3      B2  step n-1 phase B at c-5
+// with the same arithmetic per cell as k_adjoint, so both give the same bits.
+// The x-transposes are formed at use from windows of the per-column factors
+// (G = BXe L_vx, H = KXe L_px', HZ = KZe L_pz') and per-index coefficients.
+// Forward states: vx, vz after step n-2 in ring slot fa, after step n-1 in fb;
+// px, pz after step n-1 in fb, after step n in fc.
+struct AdjCol2 {
+    float lvx_n, bx_n, fvx_n;   // column c+1: L_vx, BX, fwd vx[n-1]
+    float lvz, bz;              // column c:   L_vz, BZ
+    float lpx, lpz;             //             L_px, L_pz
+    float fvz, fpx, fpz;        //             fwd vz[n-1], px[n], pz[n]
+    float kx, kz, gk, res1;     //             KX, KZ, dJ/dkappa, residual of step n
+    float gbx_j, gbz_j;         // column c-2: dJ/dbu2, dJ/dbu1
+    float fvx2;                 //             fwd vx[n-2]
+    float fvz2, fpx2, fpz2;     // column c-3: fwd vz[n-2], px[n-1], pz[n-1]
+    float res2;                 //             residual of step n-1
+};
+
+template <int WAVES, int PF, bool KZA>
+__global__ __launch_bounds__(WAVES * 64)
+void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
+                int ntiles, int step, int lam_in, int lam_out, int fa, int fb, int fc,
+                int rec1, int rec2)
+{
+    static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
+    using gcchar = const SAM2D_GLOBAL char;
+    using gchar = SAM2D_GLOBAL char;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+    for (int v = blockIdx.x; v < vtotal; v += gridDim.x) {
+        const int tile = xcd_tile(v, per_xcd);
+        if (tile >= ntiles) continue;
+        const int s = shot_of_tile(shots, nshots, tile);
+        const AdjDev &d = shots[s];
+        const int local = tile - d.tile_begin;
+        const int group = local % d.ngroups;
+        const int chunk = local / d.ngroups;
+        const int nnz = d.nnz, nnx = d.nnx;
+        const int zb = (group * WAVES + wv) * OWN_Z2;
+        if (zb >= nnz) continue;                      // wave-uniform
+        const int xa = chunk * d.chunk;
+        const int xb = min(xa + d.chunk, nnx);
+
+        const int r = zb - HALO_T2 + lane;
+        const bool rown = (lane >= HALO_T2) && (lane < HALO_T2 + OWN_Z2) && (r < nnz);
+        const unsigned rc = (unsigned)min(max(r, 0), nnz - 1);
+        const unsigned rc4 = rc * 4u;
+        const size_t N = d.ncell;
+
+        gcchar *__restrict__ Lpx = (gcchar *)(d.lam + (4 * (size_t)lam_in + 0) * N);
+        gcchar *__restrict__ Lpz = (gcchar *)(d.lam + (4 * (size_t)lam_in + 1) * N);
+        gcchar *__restrict__ Lvx = (gcchar *)(d.lam + (4 * (size_t)lam_in + 2) * N);
+        gcchar *__restrict__ Lvz = (gcchar *)(d.lam + (4 * (size_t)lam_in + 3) * N);
+        gchar *__restrict__ Opx = (gchar *)(d.lam + (4 * (size_t)lam_out + 0) * N);
+        gchar *__restrict__ Opz = (gchar *)(d.lam + (4 * (size_t)lam_out + 1) * N);
+        gchar *__restrict__ Ovx = (gchar *)(d.lam + (4 * (size_t)lam_out + 2) * N);
+        gchar *__restrict__ Ovz = (gchar *)(d.lam + (4 * (size_t)lam_out + 3) * N);
+        gcchar *__restrict__ Fpx1 = (gcchar *)(d.fwd + (4 * (size_t)fc + 0) * N);
+        gcchar *__restrict__ Fpz1 = (gcchar *)(d.fwd + (4 * (size_t)fc + 1) * N);
+        gcchar *__restrict__ Fvx1 = (gcchar *)(d.fwd + (4 * (size_t)fb + 2) * N);
+        gcchar *__restrict__ Fvz1 = (gcchar *)(d.fwd + (4 * (size_t)fb + 3) * N);
+        gcchar *__restrict__ Fpx2 = (gcchar *)(d.fwd + (4 * (size_t)fb + 0) * N);
+        gcchar *__restrict__ Fpz2 = (gcchar *)(d.fwd + (4 * (size_t)fb + 1) * N);
+        gcchar *__restrict__ Fvx2 = (gcchar *)(d.fwd + (4 * (size_t)fa + 2) * N);
+        gcchar *__restrict__ Fvz2 = (gcchar *)(d.fwd + (4 * (size_t)fa + 3) * N);
+        gcchar *__restrict__ KX = (gcchar *)d.KX;
+        gcchar *__restrict__ KZ = (gcchar *)d.KZ;
+        gcchar *__restrict__ BX = (gcchar *)d.BX;
+        gcchar *__restrict__ BZ = (gcchar *)d.BZ;
+        gchar *__restrict__ GK = (gchar *)d.gk;
+        gchar *__restrict__ GBX = (gchar *)d.gbx;
+        gchar *__restrict__ GBZ = (gchar *)d.gbz;
+        const SAM2D_GLOBAL float *__restrict__ prx = gp(d.prof_x);
+        const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
+
+        auto ld = [](gcchar *__restrict__ base, unsigned off) -> float {
+            return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
+        };
+        auto ldw = [](gchar *__restrict__ base, unsigned off) -> float {
+            return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
+        };
+        auto stf = [](gchar *__restrict__ base, unsigned off, float val) {
+            *(SAM2D_GLOBAL float *__restrict__)(base + off) = val;
+        };
+        auto coff = [&](int col) -> unsigned {
+            return (unsigned)min(max(col, 0), nnx - 1) * (unsigned)nnz * 4u + rc4;
+        };
+        auto clampc = [&](int col) -> int { return min(max(col, 0), nnx - 1); };
+        auto p0_at = [&](int i) { float a, b; dplus_coef(i, nnx, a, b); return a; };
+        auto p1_at = [&](int i) { float a, b; dplus_coef(i, nnx, a, b); return b; };
+        auto m0_at = [&](int i) { float a, b; dminus_coef(i, nnx, a, b); return a; };
+        auto m1_at = [&](int i) { float a, b; dminus_coef(i, nnx, a, b); return b; };
+
+        const float daz = prz[rc], skz = prz[nnz + rc];
+        const float dbz = prz[2 * nnz + rc], sbz = prz[3 * nnz + rc];
+        float zm0, zm1, zp0, zp1;
+        dminus_coef(r, nnz, zm0, zm1);
+        dplus_coef(r, nnz, zp0, zp1);
+        const float dtdx = d.dtdx, dtdz = d.dtdz;
+        const bool src_lane = (r == d.src_z);
+        const float srcg1 = (step < d.nts) ? gp(d.srcg)[step] : 0.0f;
+        const int src_x1 = (step < d.nts) ? d.src_x : -1;
+        const float srcg2 = (step - 1 < d.nts) ? gp(d.srcg)[step - 1] : 0.0f;
+        const int src_x2 = (step - 1 < d.nts) ? d.src_x : -1;
+        // Fast groups start at c0 in [fast_lo, fast_hi]: profiles read at
+        // columns c0-5 .. c0+4 are all ones, D+ is 4th order on c0-5 .. c0+4
+        // and D- on c0-6 .. c0+3.
+        const int fast_lo = max(d.xin_lo + 6, 8);
+        const int fast_hi = min(d.xin_hi - 4, nnx - 7);
+        const bool strip_band = d.rband != nullptr && d.rrow_lo <= zb - HALO_T2 + 63 &&
+                                d.rrow_hi >= zb - HALO_T2;
+        const bool band1 = rec1 >= 0 && strip_band, band2 = rec2 >= 0 && strip_band;
+        const bool band_lane = r >= d.rrow_lo && r <= d.rrow_hi;
+        const SAM2D_GLOBAL float *__restrict__ rb1 =
+            band1 ? gp(d.rband) + ((ptrdiff_t)rec1 * d.band_h - d.rrow_lo) * (ptrdiff_t)nnx
+                  : nullptr;
+        const SAM2D_GLOBAL float *__restrict__ rb2 =
+            band2 ? gp(d.rband) + ((ptrdiff_t)rec2 * d.band_h - d.rrow_lo) * (ptrdiff_t)nnx
+                  : nullptr;
+
+        auto load_col = [&](int c) -> AdjCol2 {
+            AdjCol2 o;
+            const unsigned o1 = coff(c + 1), o0 = coff(c), oj = coff(c - 2), o3 = coff(c - 3);
+            o.lvx_n = ld(Lvx, o1); o.bx_n = ld(BX, o1); o.fvx_n = ld(Fvx1, o1);
+            o.lvz = ld(Lvz, o0); o.bz = ld(BZ, o0);
+            o.lpx = ld(Lpx, o0); o.lpz = ld(Lpz, o0);
+            o.fvz = ld(Fvz1, o0); o.fpx = ld(Fpx1, o0); o.fpz = ld(Fpz1, o0);
+            o.kx = ld(KX, o0);
+            o.kz = KZA ? 0.0f : ld(KZ, o0);
+            o.gk = ldw(GK, o0);
+            o.gbx_j = ldw(GBX, oj); o.gbz_j = ldw(GBZ, oj);
+            o.fvx2 = ld(Fvx2, oj);
+            o.fvz2 = ld(Fvz2, o3); o.fpx2 = ld(Fpx2, o3); o.fpz2 = ld(Fpz2, o3);
+            o.res1 = -0.0f;
+            o.res2 = -0.0f;
+            if (band1) {
+                if (band_lane) o.res1 = rb1[(size_t)r * nnx + clampc(c)];
+            }
+            if (band2) {
+                if (band_lane) o.res2 = rb2[(size_t)r * nnx + clampc(c - 3)];
+            }
+            return o;
+        };
+
+        // Windows, [k & 3] relative to the unrolled base (columns held):
+        //   G1 LVX0 FVX1 BXW                      c-2 .. c+1
+        //   LVZ0 BZW KXW KZW LPX1 LPZ1 GKW H1 HZ1 PN1   c-3 .. c
+        //   G2 LVX1 LVZ1 GBXW GBZW FVX2           c-5 .. c-2
+        //   H2 HZ2 PN2                            c-6 .. c-3
+        float G1[4], LVX0[4], FVX1[4], BXW[4];
+        float LVZ0[4], BZW[4], KXW[4], KZW[4], LPX1[4], LPZ1[4], GKW[4], H1[4], HZ1[4], PN1[4];
+        float G2[4], LVX1[4], LVZ1[4], GBXW[4], GBZW[4], FVX2[4];
+        float H2[4], HZ2[4], PN2[4];
+        #pragma unroll
+        for (int k = 0; k < 4; k++) {
+            G1[k] = LVX0[k] = FVX1[k] = BXW[k] = 0.0f;
+            LVZ0[k] = BZW[k] = KXW[k] = KZW[k] = LPX1[k] = LPZ1[k] = GKW[k] = 0.0f;
+            H1[k] = HZ1[k] = PN1[k] = 0.0f;
+            G2[k] = LVX1[k] = LVZ1[k] = GBXW[k] = GBZW[k] = FVX2[k] = 0.0f;
+            H2[k] = HZ2[k] = PN2[k] = 0.0f;
+        }
+        const int c_first = xa - 4;
+        #pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int col = c_first - 2 + k;
+            const unsigned o = coff(col);
+            LVX0[k] = ld(Lvx, o);
+            FVX1[k] = ld(Fvx1, o);
+            BXW[k] = ld(BX, o);
+            G1[k] = (BXW[k] * prx[3 * nnx + clampc(col)]) * LVX0[k];
+        }
+
+        AdjCol2 q[PF];
+        #pragma unroll
+        for (int u = 0; u < PF; u++) q[u] = load_col(c_first + u);
+
+        auto column = [&](auto fast_tag, auto u_tag, int c0) {
+            constexpr bool FAST = decltype(fast_tag)::value;
+            constexpr int u = decltype(u_tag)::value;
+            constexpr int i0 = u & 3, i1 = (u + 1) & 3, i2 = (u + 2) & 3, i3 = (u + 3) & 3;
+            const int c = c0 + u;
+            AdjCol2 &cur = q[u % PF];
+
+            // ---- A1: step n, phase A at column c
+            {
+                const float lvx_n = keep(cur.lvx_n);
+                LVX0[i3] = lvx_n;
+                FVX1[i3] = keep(cur.fvx_n);
+                const float bxn = keep(cur.bx_n);
+                BXW[i3] = bxn;
+                G1[i3] = FAST ? bxn * lvx_n : (bxn * prx[3 * nnx + clampc(c + 1)]) * lvx_n;
+                float ax;
+                if (FAST) {
+                    ax = SAM2D_A0 * G1[i1] - SAM2D_A0 * G1[i2] +
+                         (SAM2D_A1 * G1[i0] - SAM2D_A1 * G1[i3]);
+                } else {
+                    ax = p0_at(c - 1) * G1[i1] - p0_at(c) * G1[i2] +
+                         (p1_at(c - 2) * G1[i0] - p1_at(c + 1) * G1[i3]);
+                }
+                const float lvz = keep(cur.lvz);
+                LVZ0[i3] = lvz;
+                const float bz = keep(cur.bz);
+                BZW[i3] = bz;
+                const float gz = (bz * sbz) * lvz;
+                const float az = zp0 * gz, bzv = zp1 * gz;
+                const float bz_u1 = lane_up1(bzv);
+                const float azz = (lane_up1(az) - az) + (lane_up1(bz_u1) - lane_dn1(bzv));
+                const float lp = ax + azz;
+                const float lpx1 = (cur.lpx + cur.res1) + lp;
+                const float lpz1 = (cur.lpz + cur.res1) + lp;
+                float xm0 = SAM2D_A0, xm1 = SAM2D_A1, skx = 1.0f, dax = 1.0f;
+                if (!FAST) {
+                    dminus_coef(c, nnx, xm0, xm1);
+                    const int cc = clampc(c);
+                    dax = prx[cc];
+                    skx = prx[nnx + cc];
+                }
+                const float f0 = cur.fvz;
+                const float f_u1 = lane_up1(f0), f_u2 = lane_up1(f_u1), f_d1 = lane_dn1(f0);
+                const float dvx = fmaf(xm0, FVX1[i2] - FVX1[i1], xm1 * (FVX1[i3] - FVX1[i0]));
+                const float dvz = fmaf(zm0, f0 - f_u1, zm1 * (f_d1 - f_u2));
+                float g = cur.gk + lpx1 * dvx * (dtdx * skx) + lpz1 * dvz * (dtdz * skz);
+                if (src_lane && c == src_x1) g += lpx1 * srcg1;
+                GKW[i3] = g;
+                LPX1[i3] = dax * lpx1;
+                LPZ1[i3] = daz * lpz1;
+                const float kx = keep(cur.kx);
+                const float kz = KZA ? kx : keep(cur.kz);
+                KXW[i3] = kx;
+                if (!KZA) KZW[i3] = kz;
+                H1[i3] = (kx * skx) * lpx1;
+                HZ1[i3] = (kz * skz) * lpz1;
+                PN1[i3] = cur.fpx + cur.fpz;
+            }
+
+            // ---- B1: step n, phase B at column j1 = c-2
+            {
+                const int j1 = c - 2;
+                float tx;
+                if (FAST) {
+                    tx = SAM2D_A0 * H1[i1] - SAM2D_A0 * H1[i2] +
+                         (SAM2D_A1 * H1[i0] - SAM2D_A1 * H1[i3]);
+                } else {
+                    tx = m0_at(j1) * H1[i1] - m0_at(j1 + 1) * H1[i2] +
+                         (m1_at(j1 - 1) * H1[i0] - m1_at(j1 + 2) * H1[i3]);
+                }
+                const float amz = zm0 * HZ1[i1], bmz = zm1 * HZ1[i1];
+                const float bm_d1 = lane_dn1(bmz);
+                const float tz = (amz - lane_dn1(amz)) + (lane_up1(bmz) - lane_dn1(bm_d1));
+                const float lvx_j = LVX0[i0], lvz_j = LVZ0[i1];
+                const float pj = PN1[i1];
+                const float q_u1 = lane_up1(pj), q_d1 = lane_dn1(pj), q_d2 = lane_dn1(q_d1);
+                float xp0 = SAM2D_A0, xp1 = SAM2D_A1, dbx = 1.0f, sbx = 1.0f;
+                if (!FAST) {
+                    dplus_coef(j1, nnx, xp0, xp1);
+                    const int jc = clampc(j1);
+                    dbx = prx[2 * nnx + jc];
+                    sbx = prx[3 * nnx + jc];
+                }
+                const float dpx = fmaf(xp0, PN1[i2] - pj, xp1 * (PN1[i3] - PN1[i0]));
+                const float dpz = fmaf(zp0, q_d1 - pj, zp1 * (q_d2 - q_u1));
+                const float ovx = fmaf(dbx, lvx_j, tx);
+                LVX1[i3] = ovx;
+                LVZ1[i3] = fmaf(dbz, lvz_j, tz);
+                GBXW[i3] = keep(cur.gbx_j) + lvx_j * dpx * (dtdx * sbx);
+                GBZW[i3] = keep(cur.gbz_j) + lvz_j * dpz * (dtdz * sbz);
+                G2[i3] = (BXW[i0] * sbx) * ovx;
+                FVX2[i3] = keep(cur.fvx2);
+            }
+
+            // ---- A2: step n-1, phase A at column c2 = c-3 (results stored)
+            {
+                const int c2 = c - 3;
+                float ax;
+                if (FAST) {
+                    ax = SAM2D_A0 * G2[i1] - SAM2D_A0 * G2[i2] +
+                         (SAM2D_A1 * G2[i0] - SAM2D_A1 * G2[i3]);
+                } else {
+                    ax = p0_at(c2 - 1) * G2[i1] - p0_at(c2) * G2[i2] +
+                         (p1_at(c2 - 2) * G2[i0] - p1_at(c2 + 1) * G2[i3]);
+                }
+                const float gz = (BZW[i0] * sbz) * LVZ1[i2];
+                const float az = zp0 * gz, bzv = zp1 * gz;
+                const float bz_u1 = lane_up1(bzv);
+                const float azz = (lane_up1(az) - az) + (lane_up1(bz_u1) - lane_dn1(bzv));
+                const float lp = ax + azz;
+                const float lpx2 = (LPX1[i0] + cur.res2) + lp;
+                const float lpz2 = (LPZ1[i0] + cur.res2) + lp;
+                float xm0 = SAM2D_A0, xm1 = SAM2D_A1, skx = 1.0f, dax = 1.0f;
+                if (!FAST) {
+                    dminus_coef(c2, nnx, xm0, xm1);
+                    const int cc = clampc(c2);
+                    dax = prx[cc];
+                    skx = prx[nnx + cc];
+                }
+                const float f0 = cur.fvz2;
+                const float f_u1 = lane_up1(f0), f_u2 = lane_up1(f_u1), f_d1 = lane_dn1(f0);
+                if (rown && c2 >= xa && c2 < xb) {
+                    const float dvx = fmaf(xm0, FVX2[i2] - FVX2[i1], xm1 * (FVX2[i3] - FVX2[i0]));
+                    const float dvz = fmaf(zm0, f0 - f_u1, zm1 * (f_d1 - f_u2));
+                    float g = GKW[i0] + lpx2 * dvx * (dtdx * skx) + lpz2 * dvz * (dtdz * skz);
+                    if (src_lane && c2 == src_x2) g += lpx2 * srcg2;
+                    const unsigned id = (unsigned)c2 * (unsigned)nnz * 4u + rc4;
+                    stf(GK, id, g);
+                    stf(Opx, id, dax * lpx2);
+                    stf(Opz, id, daz * lpz2);
+                }
+                const float kz = KZA ? KXW[i0] : KZW[i0];
+                H2[i3] = (KXW[i0] * skx) * lpx2;
+                HZ2[i3] = (kz * skz) * lpz2;
+                PN2[i3] = cur.fpx2 + cur.fpz2;
+            }
+
+            // ---- B2: step n-1, phase B at column j2 = c-5 (results stored)
+            {
+                const int j2 = c - 5;
+                float tx;
+                if (FAST) {
+                    tx = SAM2D_A0 * H2[i1] - SAM2D_A0 * H2[i2] +
+                         (SAM2D_A1 * H2[i0] - SAM2D_A1 * H2[i3]);
+                } else {
+                    tx = m0_at(j2) * H2[i1] - m0_at(j2 + 1) * H2[i2] +
+                         (m1_at(j2 - 1) * H2[i0] - m1_at(j2 + 2) * H2[i3]);
+                }
+                const float amz = zm0 * HZ2[i1], bmz = zm1 * HZ2[i1];
+                const float bm_d1 = lane_dn1(bmz);
+                const float tz = (amz - lane_dn1(amz)) + (lane_up1(bmz) - lane_dn1(bm_d1));
+                const float lvx_j = LVX1[i0], lvz_j = LVZ1[i0];
+                const float pj = PN2[i1];
+                const float q_u1 = lane_up1(pj), q_d1 = lane_dn1(pj), q_d2 = lane_dn1(q_d1);
+                if (rown && j2 >= xa && j2 < xb) {
+                    float xp0 = SAM2D_A0, xp1 = SAM2D_A1, dbx = 1.0f, sbx = 1.0f;
+                    if (!FAST) {
+                        dplus_coef(j2, nnx, xp0, xp1);
+                        const int jc = clampc(j2);
+                        dbx = prx[2 * nnx + jc];
+                        sbx = prx[3 * nnx + jc];
+                    }
+                    const float dpx = fmaf(xp0, PN2[i2] - pj, xp1 * (PN2[i3] - PN2[i0]));
+                    const float dpz = fmaf(zp0, q_d1 - pj, zp1 * (q_d2 - q_u1));
+                    const unsigned id = (unsigned)j2 * (unsigned)nnz * 4u + rc4;
+                    stf(Ovx, id, fmaf(dbx, lvx_j, tx));
+                    stf(Ovz, id, fmaf(dbz, lvz_j, tz));
+                    stf(GBX, id, GBXW[i0] + lvx_j * dpx * (dtdx * sbx));
+                    stf(GBZ, id, GBZW[i0] + lvz_j * dpz * (dtdz * sbz));
+                }
+            }
+            cur = load_col(c + PF);
+        };
+
+        using TT = std::integral_constant<bool, true>;
+        using FF = std::integral_constant<bool, false>;
+        using U0 = std::integral_constant<int, 0>;
+        using U1 = std::integral_constant<int, 1>;
+        using U2 = std::integral_constant<int, 2>;
+        using U3 = std::integral_constant<int, 3>;
+        const int c_last = xb + 4;
+        int c0 = c_first;
+        for (; c0 <= c_last && c0 < fast_lo; c0 += 4) {
+            column(FF(), U0(), c0); column(FF(), U1(), c0);
+            column(FF(), U2(), c0); column(FF(), U3(), c0);
+        }
+        const int c_fast = min(fast_hi, c_last);
+        for (; c0 <= c_fast; c0 += 4) {
+            column(TT(), U0(), c0); column(TT(), U1(), c0);
+            column(TT(), U2(), c0); column(TT(), U3(), c0);
+        }
+        for (; c0 <= c_last; c0 += 4) {
+            column(FF(), U0(), c0); column(FF(), U1(), c0);
+            column(FF(), U2(), c0); column(FF(), U3(), c0);
+        }
+    }
+}
+
 template <int W, int PF, bool KZA>
 static void launch_adj_t(dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt, int per, int nt,
-                         int step, int li, int lo, int fp, int fc)
+                         int step, int li, int lo, int fp, int fc, int rk)
 {
     hipLaunchKernelGGL((k_adjoint<W, PF, KZA>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step, li,
-                       lo, fp, fc);
+                       lo, fp, fc, rk);
 }
 
 template <int W>
 static void launch_adj_w(int pf, bool kza, dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt,
-                         int per, int nt, int step, int li, int lo, int fp, int fc)
+                         int per, int nt, int step, int li, int lo, int fp, int fc, int rk)
 {
-    if (pf == 2 && kza) launch_adj_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc);
-    else if (pf == 2) launch_adj_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc);
-    else if (kza) launch_adj_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc);
-    else launch_adj_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc);
+    if (pf == 2 && kza) launch_adj_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk);
+    else if (pf == 2) launch_adj_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk);
+    else if (kza) launch_adj_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk);
+    else launch_adj_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk);
 }
 
 hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
                                bool kz_alias, int step, int lam_in, int lam_out, int fwd_prev,
-                               int fwd_cur, hipStream_t stream)
+                               int fwd_cur, int rec_k, hipStream_t stream)
 {
     int waves, chunk, pf;
     variant_shape_pf(variant, &waves, &chunk, &pf);
@@ -340,10 +774,48 @@ hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int
     const int vtotal = per * 8;
     const dim3 g(vtotal);
     switch (waves) {
-    case 1: launch_adj_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur); break;
-    case 2: launch_adj_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur); break;
-    case 4: launch_adj_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur); break;
-    case 8: launch_adj_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur); break;
+    case 1: launch_adj_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
+    case 2: launch_adj_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
+    case 4: launch_adj_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
+    case 8: launch_adj_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int W, int PF, bool KZA>
+static void launch_adj2_t(dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt, int per,
+                          int nt, int step, int li, int lo, int fa, int fb, int fc, int r1, int r2)
+{
+    hipLaunchKernelGGL((k_adjoint2<W, PF, KZA>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step,
+                       li, lo, fa, fb, fc, r1, r2);
+}
+
+template <int W>
+static void launch_adj2_w(int pf, bool kza, dim3 g, hipStream_t s, const AdjDev *d, int ns,
+                          int vt, int per, int nt, int step, int li, int lo, int fa, int fb, int fc,
+                          int r1, int r2)
+{
+    if (pf == 2 && kza) launch_adj2_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2);
+    else if (pf == 2) launch_adj2_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2);
+    else if (kza) launch_adj2_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2);
+    else launch_adj2_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2);
+}
+
+hipError_t launch_adjoint_step2(int variant, const AdjDev *d_adj, int nshots, int ntiles,
+                                bool kz_alias, int step, int lam_in, int lam_out, int fa, int fb,
+                                int fc, int rec1, int rec2, hipStream_t stream)
+{
+    int waves, chunk, pf;
+    variant_shape_pf(variant, &waves, &chunk, &pf);
+    const int per = (ntiles + 7) / 8;
+    const int vtotal = per * 8;
+    const dim3 g(vtotal);
+    switch (waves) {
+    case 1: launch_adj2_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2); break;
+    case 2: launch_adj2_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2); break;
+    case 4: launch_adj2_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2); break;
+    case 8: launch_adj2_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

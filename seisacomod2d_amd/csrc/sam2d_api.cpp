@@ -698,6 +698,9 @@ int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value)
     else if (k == "hbm_budget_gb") {
         if (value < 1) return fail(SAM2D_ERR_PARAM, "hbm_budget_gb must be >= 1");
         ctx->hbm_budget_gb = value;
+    } else if (k == "adjoint_steps") {
+        if (value != 1 && value != 2) return fail(SAM2D_ERR_PARAM, "adjoint_steps must be 1 or 2");
+        ctx->adjoint_steps = value;
     }
     else return fail(SAM2D_ERR_PARAM, "unknown option '%s'", key);
     return SAM2D_OK;

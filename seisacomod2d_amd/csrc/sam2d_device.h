@@ -10,7 +10,9 @@ constexpr int OWN_Z = 58;
 constexpr int HALO_T = 3;
 constexpr int HALO_B = 64 - OWN_Z - HALO_T;
 // Kernels that advance T > 1 time steps per pass carry 3T halo rows on each
-// side and own 64 - 6T rows.
+// side and own 64 - 6T rows (T = 2: the two-step adjoint).
+constexpr int OWN_Z2 = 52;
+constexpr int HALO_T2 = 6;
 // Band slices of the receiver-row pressure buffer: one per intermediate step
 // of a launch (up to five steps per launch).
 constexpr int BAND_SLICES = 4;
@@ -54,7 +56,8 @@ struct AdjDev {
     int ngroups, nchunks, chunk, tile_begin;
     int src_x, src_z, nts, real_nt;
     int xin_lo, xin_hi;    // columns where all four x-profiles are exactly 1
-    int ncells, pad0;      // distinct receiver cells
+    int ncells;            // distinct receiver cells
+    int band_h, rrow_lo, rrow_hi;   // receiver rows (rband layout)
     float dtdx, dtdz;      // fd_dt/dx, fd_dt/dz as the coefficients use them
     const float *srcg;     // [nts] d(source increment)/d(kappa_s) = fd_dt*w[n]/(dx*dz)
     const float *KX, *KZ, *BX, *BZ;
@@ -69,15 +72,31 @@ struct AdjDev {
     const int *rptr;       // [ncells+1] receivers of each cell (CSR)
     const int *rid;        // [ng] receiver index of each entry, ascending per cell
     const float *res;      // [ng][real_nt] adjoint source dJ/dseis
+    const float *rband;    // [real_nt][band_h][nnx] residual per receiver cell
+                           // (-0 elsewhere), or NULL: injected by k_inject
 };
 
 // One backward step n of every staged shot: adjoint state lam_in -> lam_out,
-// forward states S(n-1) in ring slot fwd_prev and S(n) in fwd_cur.
+// forward states S(n-1) in ring slot fwd_prev and S(n) in fwd_cur; rec_k >= 0
+// injects residual sample rec_k of the shots that carry an rband.
 // kz_alias: every staged shot's KZ is its KX.
 hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
                                bool kz_alias, int step, int lam_in, int lam_out, int fwd_prev,
-                               int fwd_cur, hipStream_t stream);
-// Residual sample rec_k into adjoint slot lam before a backward step.
+                               int fwd_cur, int rec_k, hipStream_t stream);
+// Backward steps n and n-1 in one pass (all shots must carry an rband or no
+// receivers): forward vx, vz after step n-2 in ring slot fa, after n-1 in fb;
+// px, pz after n-1 in fb, after n in fc; residual samples rec1 (step n) and
+// rec2 (step n-1).  Tiles of OWN_Z2-row strips.
+hipError_t launch_adjoint_step2(int variant, const AdjDev *d_adj, int nshots, int ntiles,
+                                bool kz_alias, int step, int lam_in, int lam_out, int fa, int fb,
+                                int fc, int rec1, int rec2, hipStream_t stream);
+// Dense residual band [real_nt][band_cells]: element (k, i) is the sum, in
+// rid order, of res[rid[e] * real_nt + k] over e in [rptr[u], rptr[u+1]) with
+// u = bmap[i], starting from -0; -0 where bmap[i] < 0.
+hipError_t launch_band(const int *bmap, const int *rptr, const int *rid, const float *res,
+                       int real_nt, int band_cells, float *band, hipStream_t stream);
+// Residual sample rec_k into adjoint slot lam before a backward step (shots
+// without an rband).
 hipError_t launch_inject(const AdjDev *d_adj, int nshots, int max_cells, int lam, int rec_k,
                          hipStream_t stream);
 

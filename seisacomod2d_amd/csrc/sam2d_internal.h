@@ -108,6 +108,7 @@ struct sam2d_ctx {
     int max_batch = 0;
     int grid_cap = 0;
     int hbm_budget_gb = 64;                // device memory a gradient batch may use
+    int adjoint_steps = 2;                 // backward steps per adjoint launch (1 or 2)
     std::vector<double> tune_ms;           // per variant, from the autotuner
     sam2d_stats stats = sam2d_stats();
     sam2d::AdjointState *adj = nullptr;
