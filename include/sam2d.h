@@ -152,7 +152,9 @@ int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
  * or, for gradients, as many as fit "hbm_budget_gb" (default 64 GiB of device
  * memory for ring, checkpoints and adjoint state)), "adjoint_steps" (backward
  * steps per adjoint kernel pass in sam2d_gradient, 1 or 2, default 2; both
- * give the same bits). */
+ * give the same bits), "adjoint_variant" (tile shape of the adjoint kernels,
+ * the single-step form of the given variant index; -1 = automatic, the
+ * default). */
 int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value);
 
 /* Project-written autotuner: times every kernel variant (z-strips per

@@ -124,8 +124,14 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     AdjointState &A = *ctx->adj;
     const TimeAxis &ta = ctx->ta;
     const int ns = (int)plans.size();
-    // Every forward state is needed in the ring: single-step kernels only.
-    const int variant = variant_single_step(pick_variant(ctx));
+    // Tile shape of the adjoint and of the single-step recomputation: the
+    // "adjoint_variant" option, else the fastest shape measured for the C4
+    // gradient on MI355X (two steps: 2 waves x 32 columns, prefetch 4; one
+    // step: 1 wave x 64 columns, prefetch 2; within 6 % of each other for
+    // 32..128 columns).
+    const int variant = ctx->adjoint_variant >= 0 ? ctx->adjoint_variant
+                        : ctx->adjoint_steps == 2 ? variant_index(2, 32, 4)
+                                                  : variant_index(1, 64, 2);
     int ntiles = 0;
     SAM2D_TRY(stage_batch(ctx, plans, variant, &ntiles, k + 1));
     if ((int)A.bufs.size() < ns) A.bufs.resize(ns);

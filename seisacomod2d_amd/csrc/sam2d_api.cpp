@@ -701,6 +701,9 @@ int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value)
     } else if (k == "adjoint_steps") {
         if (value != 1 && value != 2) return fail(SAM2D_ERR_PARAM, "adjoint_steps must be 1 or 2");
         ctx->adjoint_steps = value;
+    } else if (k == "adjoint_variant") {
+        if (value >= num_variants()) return fail(SAM2D_ERR_PARAM, "variant %d out of range", value);
+        ctx->adjoint_variant = value < 0 ? -1 : variant_single_step(value);
     }
     else return fail(SAM2D_ERR_PARAM, "unknown option '%s'", key);
     return SAM2D_OK;

@@ -111,6 +111,8 @@ int variant_steps(int v);
 int variant_own_rows(int v);
 // The single-step variant with the same workgroup and chunk shape.
 int variant_single_step(int v);
+// Single-step variant with this shape (ordinary stores), or -1.
+int variant_index(int waves, int chunk, int pf);
 
 // T = variant_steps(variant) time steps (step .. step+T-1) of every staged
 // shot, in_slot -> out_slot.  band_mask bit j keeps the pressure after step

@@ -109,6 +109,7 @@ struct sam2d_ctx {
     int grid_cap = 0;
     int hbm_budget_gb = 64;                // device memory a gradient batch may use
     int adjoint_steps = 2;                 // backward steps per adjoint launch (1 or 2)
+    int adjoint_variant = -1;              // single-step variant for adjoint tiles, -1 = follow
     std::vector<double> tune_ms;           // per variant, from the autotuner
     sam2d_stats stats = sam2d_stats();
     sam2d::AdjointState *adj = nullptr;

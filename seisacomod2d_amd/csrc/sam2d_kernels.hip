@@ -309,6 +309,16 @@ int variant_single_step(int v)
     return wi * nC + ci;   // prefetch 2, ordinary stores
 }
 
+int variant_index(int waves, int chunk, int pf)
+{
+    int wi = -1, ci = -1, pi = -1;
+    for (int i = 0; i < nW; i++) if (kWaves[i] == waves) wi = i;
+    for (int i = 0; i < nC; i++) if (kChunks[i] == chunk) ci = i;
+    for (int i = 0; i < nP; i++) if (kPF[i] == pf) pi = i;
+    if (wi < 0 || ci < 0 || pi < 0) return -1;
+    return (pi * nW + wi) * nC + ci;   // ordinary stores
+}
+
 void variant_name(int v, char *buf, int n)
 {
     int w, c, pf, nts, steps;
