@@ -126,11 +126,11 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     const int ns = (int)plans.size();
     // Tile shape of the adjoint and of the single-step recomputation: the
     // "adjoint_variant" option, else the fastest shape measured for the C4
-    // gradient on MI355X (two steps: 2 waves x 32 columns, prefetch 4; one
-    // step: 1 wave x 64 columns, prefetch 2; within 6 % of each other for
-    // 32..128 columns).
+    // gradient on MI355X (two steps: 4 waves x 32 columns, prefetch 2, best
+    // with 4 and with 32 shots resident; one step: 1 wave x 64 columns,
+    // prefetch 2; shapes of 16..64 columns are within 15 %).
     const int variant = ctx->adjoint_variant >= 0 ? ctx->adjoint_variant
-                        : ctx->adjoint_steps == 2 ? variant_index(2, 32, 4)
+                        : ctx->adjoint_steps == 2 ? variant_index(4, 32, 2)
                                                   : variant_index(1, 64, 2);
     int ntiles = 0;
     SAM2D_TRY(stage_batch(ctx, plans, variant, &ntiles, k + 1));
@@ -327,15 +327,20 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
             for (; n + 2 < n1; n += 3)
                 HIP_TRY(launch_stepn_ring(fwd_variant, fwd_desc, ns, fwd_tiles, ctx->grid_cap, kza,
                                           n, n - n0, ctx->stream));
+        // Ring slots 1, 2, 4, 5, ... below the last three-step launch hold p
+        // in place of px and pz.
+        const int ring_end = n - n0;
+        auto psum = [&](int slot) -> int { return slot <= ring_end && slot % 3 != 0 ? 1 : 0; };
         for (; n < n1; n++)
             HIP_TRY(launch_step(variant, ctx->d_desc, ns, ntiles, ctx->grid_cap, n, n - n0,
                                 n - n0 + 1, -1, ctx->stream));
         int n_top = n1 - 1;
         if (two_step)
             for (; n_top - 1 >= n0; n_top -= 2) {
+                const int fc = n_top - n0 + 1, fb = n_top - n0;
                 HIP_TRY(launch_adjoint_step2(variant, A.d_desc2, ns, ntiles2, kza, n_top, lam,
-                                             lam ^ 1, n_top - n0 - 1, n_top - n0, n_top - n0 + 1,
-                                             rec_sample(ta, n_top), rec_sample(ta, n_top - 1),
+                                             lam ^ 1, fb - 1, fb, fc, rec_sample(ta, n_top),
+                                             rec_sample(ta, n_top - 1), psum(fc) | psum(fb) << 1,
                                              ctx->stream));
                 lam ^= 1;
             }
@@ -343,7 +348,8 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
             if (!all_band)
                 HIP_TRY(launch_inject(A.d_desc, ns, max_cells, lam, rec_sample(ta, n), ctx->stream));
             HIP_TRY(launch_adjoint_step(variant, A.d_desc, ns, ntiles, kza, n, lam, lam ^ 1, n - n0,
-                                        n - n0 + 1, rec_sample(ta, n), ctx->stream));
+                                        n - n0 + 1, rec_sample(ta, n), psum(n - n0 + 1),
+                                        ctx->stream));
             lam ^= 1;
         }
     }

@@ -116,7 +116,7 @@ template <int WAVES, int PF, bool KZA>
 __global__ __launch_bounds__(WAVES * 64)
 void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
                int ntiles, int step, int lam_in, int lam_out, int fwd_prev, int fwd_cur,
-               int rec_k)
+               int rec_k, int psum)
 {
     static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
     using gcchar = const SAM2D_GLOBAL char;
@@ -154,8 +154,11 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         gchar *__restrict__ Opz = (gchar *)(d.lam + (4 * (size_t)lam_out + 1) * N);
         gchar *__restrict__ Ovx = (gchar *)(d.lam + (4 * (size_t)lam_out + 2) * N);
         gchar *__restrict__ Ovz = (gchar *)(d.lam + (4 * (size_t)lam_out + 3) * N);
+        // psum & 1: slot fwd_cur holds p in its px field and no pz; the pz
+        // loads then read the px field again (a cache hit) and are not used.
+        const bool ps_cur = psum & 1;
         gcchar *__restrict__ Fpx = (gcchar *)(d.fwd + (4 * (size_t)fwd_cur + 0) * N);
-        gcchar *__restrict__ Fpz = (gcchar *)(d.fwd + (4 * (size_t)fwd_cur + 1) * N);
+        gcchar *__restrict__ Fpz = (gcchar *)(d.fwd + (4 * (size_t)fwd_cur + (ps_cur ? 0 : 1)) * N);
         gcchar *__restrict__ Fvx = (gcchar *)(d.fwd + (4 * (size_t)fwd_prev + 2) * N);
         gcchar *__restrict__ Fvz = (gcchar *)(d.fwd + (4 * (size_t)fwd_prev + 3) * N);
         gcchar *__restrict__ KX = (gcchar *)d.KX;
@@ -310,7 +313,7 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             BM[i3] = xm1 * hx;
             AMZ[i3] = zm0 * hz;
             BMZ[i3] = zm1 * hz;
-            PN[i3] = cur.fpx + cur.fpz;
+            PN[i3] = ps_cur ? cur.fpx : cur.fpx + cur.fpz;
 
             // ---- phase B at column j = c - 2
             const int j = c - 2;
@@ -393,7 +396,7 @@ template <int WAVES, int PF, bool KZA>
 __global__ __launch_bounds__(WAVES * 64)
 void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
                 int ntiles, int step, int lam_in, int lam_out, int fa, int fb, int fc,
-                int rec1, int rec2)
+                int rec1, int rec2, int psum)
 {
     static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
     using gcchar = const SAM2D_GLOBAL char;
@@ -429,12 +432,15 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         gchar *__restrict__ Opz = (gchar *)(d.lam + (4 * (size_t)lam_out + 1) * N);
         gchar *__restrict__ Ovx = (gchar *)(d.lam + (4 * (size_t)lam_out + 2) * N);
         gchar *__restrict__ Ovz = (gchar *)(d.lam + (4 * (size_t)lam_out + 3) * N);
+        // psum bit 0 (slot fc) and bit 1 (slot fb): the slot holds p in its px
+        // field and no pz; the pz loads then read the px field again.
+        const bool ps_c = psum & 1, ps_b = (psum >> 1) & 1;
         gcchar *__restrict__ Fpx1 = (gcchar *)(d.fwd + (4 * (size_t)fc + 0) * N);
-        gcchar *__restrict__ Fpz1 = (gcchar *)(d.fwd + (4 * (size_t)fc + 1) * N);
+        gcchar *__restrict__ Fpz1 = (gcchar *)(d.fwd + (4 * (size_t)fc + (ps_c ? 0 : 1)) * N);
         gcchar *__restrict__ Fvx1 = (gcchar *)(d.fwd + (4 * (size_t)fb + 2) * N);
         gcchar *__restrict__ Fvz1 = (gcchar *)(d.fwd + (4 * (size_t)fb + 3) * N);
         gcchar *__restrict__ Fpx2 = (gcchar *)(d.fwd + (4 * (size_t)fb + 0) * N);
-        gcchar *__restrict__ Fpz2 = (gcchar *)(d.fwd + (4 * (size_t)fb + 1) * N);
+        gcchar *__restrict__ Fpz2 = (gcchar *)(d.fwd + (4 * (size_t)fb + (ps_b ? 0 : 1)) * N);
         gcchar *__restrict__ Fvx2 = (gcchar *)(d.fwd + (4 * (size_t)fa + 2) * N);
         gcchar *__restrict__ Fvz2 = (gcchar *)(d.fwd + (4 * (size_t)fa + 3) * N);
         gcchar *__restrict__ KX = (gcchar *)d.KX;
@@ -604,7 +610,7 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                 if (!KZA) KZW[i3] = kz;
                 H1[i3] = (kx * skx) * lpx1;
                 HZ1[i3] = (kz * skz) * lpz1;
-                PN1[i3] = cur.fpx + cur.fpz;
+                PN1[i3] = ps_c ? cur.fpx : cur.fpx + cur.fpz;
             }
 
             // ---- B1: step n, phase B at column j1 = c-2
@@ -682,7 +688,7 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                 const float kz = KZA ? KXW[i0] : KZW[i0];
                 H2[i3] = (KXW[i0] * skx) * lpx2;
                 HZ2[i3] = (kz * skz) * lpz2;
-                PN2[i3] = cur.fpx2 + cur.fpz2;
+                PN2[i3] = ps_b ? cur.fpx2 : cur.fpx2 + cur.fpz2;
             }
 
             // ---- B2: step n-1, phase B at column j2 = c-5 (results stored)
@@ -748,25 +754,26 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
 
 template <int W, int PF, bool KZA>
 static void launch_adj_t(dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt, int per, int nt,
-                         int step, int li, int lo, int fp, int fc, int rk)
+                         int step, int li, int lo, int fp, int fc, int rk, int ps)
 {
     hipLaunchKernelGGL((k_adjoint<W, PF, KZA>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step, li,
-                       lo, fp, fc, rk);
+                       lo, fp, fc, rk, ps);
 }
 
 template <int W>
 static void launch_adj_w(int pf, bool kza, dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt,
-                         int per, int nt, int step, int li, int lo, int fp, int fc, int rk)
+                         int per, int nt, int step, int li, int lo, int fp, int fc, int rk,
+                         int ps)
 {
-    if (pf == 2 && kza) launch_adj_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk);
-    else if (pf == 2) launch_adj_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk);
-    else if (kza) launch_adj_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk);
-    else launch_adj_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk);
+    if (pf == 2 && kza) launch_adj_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps);
+    else if (pf == 2) launch_adj_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps);
+    else if (kza) launch_adj_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps);
+    else launch_adj_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps);
 }
 
 hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
                                bool kz_alias, int step, int lam_in, int lam_out, int fwd_prev,
-                               int fwd_cur, int rec_k, hipStream_t stream)
+                               int fwd_cur, int rec_k, int psum, hipStream_t stream)
 {
     int waves, chunk, pf;
     variant_shape_pf(variant, &waves, &chunk, &pf);
@@ -774,10 +781,10 @@ hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int
     const int vtotal = per * 8;
     const dim3 g(vtotal);
     switch (waves) {
-    case 1: launch_adj_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
-    case 2: launch_adj_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
-    case 4: launch_adj_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
-    case 8: launch_adj_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k); break;
+    case 1: launch_adj_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum); break;
+    case 2: launch_adj_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum); break;
+    case 4: launch_adj_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum); break;
+    case 8: launch_adj_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -785,26 +792,27 @@ hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int
 
 template <int W, int PF, bool KZA>
 static void launch_adj2_t(dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt, int per,
-                          int nt, int step, int li, int lo, int fa, int fb, int fc, int r1, int r2)
+                          int nt, int step, int li, int lo, int fa, int fb, int fc, int r1, int r2,
+                          int ps)
 {
     hipLaunchKernelGGL((k_adjoint2<W, PF, KZA>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step,
-                       li, lo, fa, fb, fc, r1, r2);
+                       li, lo, fa, fb, fc, r1, r2, ps);
 }
 
 template <int W>
 static void launch_adj2_w(int pf, bool kza, dim3 g, hipStream_t s, const AdjDev *d, int ns,
                           int vt, int per, int nt, int step, int li, int lo, int fa, int fb, int fc,
-                          int r1, int r2)
+                          int r1, int r2, int ps)
 {
-    if (pf == 2 && kza) launch_adj2_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2);
-    else if (pf == 2) launch_adj2_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2);
-    else if (kza) launch_adj2_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2);
-    else launch_adj2_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2);
+    if (pf == 2 && kza) launch_adj2_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps);
+    else if (pf == 2) launch_adj2_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps);
+    else if (kza) launch_adj2_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps);
+    else launch_adj2_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps);
 }
 
 hipError_t launch_adjoint_step2(int variant, const AdjDev *d_adj, int nshots, int ntiles,
                                 bool kz_alias, int step, int lam_in, int lam_out, int fa, int fb,
-                                int fc, int rec1, int rec2, hipStream_t stream)
+                                int fc, int rec1, int rec2, int psum, hipStream_t stream)
 {
     int waves, chunk, pf;
     variant_shape_pf(variant, &waves, &chunk, &pf);
@@ -812,10 +820,10 @@ hipError_t launch_adjoint_step2(int variant, const AdjDev *d_adj, int nshots, in
     const int vtotal = per * 8;
     const dim3 g(vtotal);
     switch (waves) {
-    case 1: launch_adj2_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2); break;
-    case 2: launch_adj2_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2); break;
-    case 4: launch_adj2_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2); break;
-    case 8: launch_adj2_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2); break;
+    case 1: launch_adj2_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum); break;
+    case 2: launch_adj2_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum); break;
+    case 4: launch_adj2_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum); break;
+    case 8: launch_adj2_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

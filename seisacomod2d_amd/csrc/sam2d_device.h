@@ -79,17 +79,19 @@ struct AdjDev {
 // One backward step n of every staged shot: adjoint state lam_in -> lam_out,
 // forward states S(n-1) in ring slot fwd_prev and S(n) in fwd_cur; rec_k >= 0
 // injects residual sample rec_k of the shots that carry an rband.
-// kz_alias: every staged shot's KZ is its KX.
+// kz_alias: every staged shot's KZ is its KX.  psum bit 0: fwd_cur is a p-sum
+// slot (launch_stepn_ring).
 hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
                                bool kz_alias, int step, int lam_in, int lam_out, int fwd_prev,
-                               int fwd_cur, int rec_k, hipStream_t stream);
+                               int fwd_cur, int rec_k, int psum, hipStream_t stream);
 // Backward steps n and n-1 in one pass (all shots must carry an rband or no
 // receivers): forward vx, vz after step n-2 in ring slot fa, after n-1 in fb;
 // px, pz after n-1 in fb, after n in fc; residual samples rec1 (step n) and
-// rec2 (step n-1).  Tiles of OWN_Z2-row strips.
+// rec2 (step n-1).  psum bit 0: fc is a p-sum slot, bit 1: fb is.  Tiles of
+// OWN_Z2-row strips.
 hipError_t launch_adjoint_step2(int variant, const AdjDev *d_adj, int nshots, int ntiles,
                                 bool kz_alias, int step, int lam_in, int lam_out, int fa, int fb,
-                                int fc, int rec1, int rec2, hipStream_t stream);
+                                int fc, int rec1, int rec2, int psum, hipStream_t stream);
 // Dense residual band [real_nt][band_cells]: element (k, i) is the sum, in
 // rid order, of res[rid[e] * real_nt + k] over e in [rptr[u], rptr[u+1]) with
 // u = bmap[i], starting from -0; -0 where bmap[i] < 0.
@@ -126,7 +128,9 @@ hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int nti
                         int band_w, const int *band_samples, int state_sample, int max_ng,
                         hipStream_t stream);
 // The recomputation form: the states after all three steps of a three-step
-// variant are stored, into slots in_slot+1 .. in_slot+3 (no receivers).
+// variant are stored, into slots in_slot+1 .. in_slot+3 (no receivers).  The
+// intermediate slots in_slot+1 and in_slot+2 hold p = px + pz in the px field
+// and no pz (a "p-sum" slot); in_slot+3 is a full state.
 bool ring_supported(int variant);
 hipError_t launch_stepn_ring(int variant, const ShotDev *d_shots, int nshots, int ntiles,
                              int grid_cap, bool kz_alias, int step, int in_slot, hipStream_t stream);

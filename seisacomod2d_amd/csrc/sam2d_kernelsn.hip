@@ -49,7 +49,9 @@ struct ColN {
 // RING: the states after the intermediate steps are stored as well, into
 // state slots in_slot+1 .. in_slot+T-1 (out_slot = in_slot+T): the
 // recomputation of a checkpoint segment for the adjoint, which needs every
-// state, at the multi-step kernel's read traffic.
+// state, at the multi-step kernel's read traffic.  The adjoint reads only
+// p = px + pz of a state, so an intermediate slot holds p (the same rounded
+// sum) in place of px, vx and vz, and no pz: 12 B per cell instead of 16.
 template <int T, int WAVES, int PF, bool KZA, bool RING>
 __global__ __launch_bounds__(WAVES * 64)
 void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_xcd, int ntiles,
@@ -243,8 +245,7 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                         }
                         if (RING && rown && cp >= xa && cp < xb) {
                             const unsigned id = (unsigned)cp * (unsigned)nnz * 4u + rc4;
-                            stf(mid[t][0], id, pxn);
-                            stf(mid[t][1], id, pzn);
+                            stf(mid[t][0], id, P[t][i3]);
                         }
                     } else if (rown && cp >= xa && cp < xb) {
                         const unsigned id = (unsigned)cp * (unsigned)nnz * 4u + rc4;

@@ -114,6 +114,13 @@ def test_multistep_forward_sweep_bitwise():
                 assert np.array_equal(g[0], ref[0]) and np.array_equal(g[1], ref[1]), (name, k)
                 for a, b in zip(g[2], ref[2]):
                     assert np.array_equal(a, b), (name, k)
+        # The three-step recomputation stores p = px + pz for its intermediate
+        # states; both adjoint kernels read those slots.
+        m.set_option("variant", names.index("threestep_waves2_chunk64"))
+        m.set_option("adjoint_steps", 1)
+        for k in (12, 8, 7):
+            g = m.gradient(shots, adj, ckpt_k=k)
+            assert np.array_equal(g[0], ref[0]) and np.array_equal(g[1], ref[1]), k
 
 
 def _deep_setup(dz=10.0, T=0.16):
