@@ -164,9 +164,13 @@ def bench_plain(args, sam):
             m.set_option("graph", 0)
         if args.variant >= 0:
             m.set_option("variant", args.variant)
+            m.set_option("tile_order", max(args.tile_order, 0))
         else:
             best = m.autotune(steps=6)
-            print(f"# autotune picked {sam.variants()[best]}", file=sys.stderr, flush=True)
+            if args.tile_order >= 0:
+                m.set_option("tile_order", args.tile_order)
+            print(f"# autotune picked {sam.variants()[best]}, tile order "
+                  f"{m.get_option('tile_order')}", file=sys.stderr, flush=True)
             if args.tuning:
                 for name, ms in zip(sam.variants(), m.tuning()):
                     print(f"#   {name:28s} {ms * 1e3:8.1f} us/step", file=sys.stderr)
@@ -187,6 +191,7 @@ def bench_plain(args, sam):
         hip.sync()
         wall = time.perf_counter() - t0
         variant = sam.variants()[st["variant"]]
+        tile_order = m.get_option("tile_order")
         spl = st["steps_per_launch"]
         used_graph = st["used_graph"]
         hip.free(seis)
@@ -227,7 +232,8 @@ def bench_plain(args, sam):
         "data": "synthetic two-layer model (analytic), 64 shot positions cycled",
         "config": {"workload": "config1_plain_forward_5000x5000_nt4000_npml50",
                    "grid": [NX, NZ], "padded": [m.nnx, m.nnz], "nt": m.fd_nt, "receivers": ng,
-                   "shots_per_job": NSHOTS, "kernel_variant": variant, "hipgraph": used_graph,
+                   "shots_per_job": NSHOTS, "kernel_variant": variant, "tile_order": tile_order,
+                   "hipgraph": used_graph,
                    "profile_run": args.profile_steps > 0},
         "roofline": roof,
         "job_seconds_64_shots": round(wall / args.steps * NSHOTS, 2),
@@ -407,6 +413,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--tile-order", type=int, default=-1,
+                    help="chunk-column stride of the multi-step kernels (-1: autotuned, or 0 "
+                         "with --variant)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2, choices=(1, 2),
                     help="gradient workload: backward steps per adjoint pass")

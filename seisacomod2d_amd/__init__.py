@@ -79,6 +79,7 @@ def lib():
         L.sam2d_gradient.argtypes = [vp, P(_Shot), ctypes.c_int, i32p, f32p, ctypes.c_int,
                                      f32p, f32p, f32p]
         L.sam2d_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+        L.sam2d_get_option.argtypes = [vp, ctypes.c_char_p, P(ctypes.c_int)]
         L.sam2d_autotune.argtypes = [vp, i32p, ctypes.c_int, ctypes.c_int, i32p]
         L.sam2d_num_variants.restype = ctypes.c_int
         L.sam2d_variant_name.argtypes = [ctypes.c_int]
@@ -203,6 +204,11 @@ class Modeller:
 
     def set_option(self, key: str, value: int):
         _check(lib().sam2d_set_option(self._h, key.encode(), int(value)))
+
+    def get_option(self, key: str) -> int:
+        v = ctypes.c_int(0)
+        _check(lib().sam2d_get_option(self._h, key.encode(), ctypes.byref(v)))
+        return v.value
 
     def stats(self):
         s = _Stats()

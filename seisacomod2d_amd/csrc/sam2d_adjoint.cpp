@@ -280,6 +280,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     int fwd_variant = pick_variant(ctx);
     const ShotDev *fwd_desc = ctx->d_desc;
     int fwd_tiles = ntiles;
+    int fwd_groups = ctx->staged_desc[0].ngroups;
     if (variant_steps(fwd_variant) > 1) {
         int w2, c2;
         variant_shape(fwd_variant, &w2, &c2);
@@ -299,6 +300,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         SAM2D_TRY(upload(A.d_fwd, d2.data(), sizeof(ShotDev) * ns, ctx->stream));
         fwd_desc = A.d_fwd;
         fwd_tiles = t2;
+        fwd_groups = d2[0].ngroups;
     } else {
         fwd_variant = variant;
     }
@@ -311,7 +313,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     for (int m = 0; m < nseg; m++) {
         SAM2D_TRY(save_or_restore(m, cur, true));
         const int n0 = m * k, n1 = std::min(ta.fd_nt, n0 + k);
-        SAM2D_TRY(enqueue_steps_on(ctx, fwd_desc, ns, fwd_tiles, fwd_variant, n0, n1,
+        SAM2D_TRY(enqueue_steps_on(ctx, fwd_desc, fwd_groups, ns, fwd_tiles, fwd_variant, n0, n1,
                                    want_seis ? max_ng : 0, cur, &cur));
     }
     // 2. backward by segments
@@ -326,7 +328,8 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         if (ring_supported(fwd_variant))
             for (; n + 2 < n1; n += 3)
                 HIP_TRY(launch_stepn_ring(fwd_variant, fwd_desc, ns, fwd_tiles, ctx->grid_cap, kza,
-                                          n, n - n0, ctx->stream));
+                                          n, n - n0, TileOrder{fwd_groups, ctx->tile_order},
+                                          ctx->stream));
         // Ring slots 1, 2, 4, 5, ... below the last three-step launch hold p
         // in place of px and pz.
         const int ring_end = n - n0;

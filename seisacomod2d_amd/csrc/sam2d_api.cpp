@@ -315,13 +315,15 @@ static int rec_after(const TimeAxis &ta, int n)
 int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int n0, int n1, int max_ng,
                   int start_slot, int *end_slot)
 {
-    return enqueue_steps_on(ctx, ctx->d_desc, nshots, ntiles, variant, n0, n1, max_ng, start_slot,
-                            end_slot);
+    const int ngroups = ctx->staged_desc.empty() ? 0 : ctx->staged_desc[0].ngroups;
+    return enqueue_steps_on(ctx, ctx->d_desc, ngroups, nshots, ntiles, variant, n0, n1, max_ng,
+                            start_slot, end_slot);
 }
 
-int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int nshots, int ntiles, int variant,
-                     int n0, int n1, int max_ng, int start_slot, int *end_slot)
+int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int ngroups, int nshots, int ntiles,
+                     int variant, int n0, int n1, int max_ng, int start_slot, int *end_slot)
 {
+    const TileOrder order{ngroups, ctx->tile_order};
     const TimeAxis &ta = ctx->ta;
     int cur = start_slot;
     int n = n0;
@@ -340,7 +342,7 @@ int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int nshots, int ntiles
                 if (rec[j] >= 0) mask |= 1 << j;
             }
             HIP_TRY(launch_stepn(variant, desc, nshots, ntiles, ctx->grid_cap, kza, n, cur, cur ^ 1,
-                                 mask, band_w, pend, pend_state, max_ng, ctx->stream));
+                                 mask, band_w, pend, pend_state, max_ng, order, ctx->stream));
             for (int j = 0; j < BAND_SLICES; j++) pend[j] = rec[j];
             pend_state = rec_after(ta, n + T - 1);
             cur ^= 1;
@@ -387,7 +389,7 @@ int run_time_loop(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int max_n
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     bool used_graph = false;
     if (ctx->use_graph) {
-        GraphKey key{nshots, ntiles, variant, max_ng, ctx->grid_cap};
+        GraphKey key{nshots, ntiles, variant, max_ng, ctx->grid_cap, ctx->tile_order};
         hipGraphExec_t exec = nullptr;
         auto it = ctx->graphs.find(key);
         if (it != ctx->graphs.end()) exec = it->second;
@@ -701,10 +703,31 @@ int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value)
     } else if (k == "adjoint_steps") {
         if (value != 1 && value != 2) return fail(SAM2D_ERR_PARAM, "adjoint_steps must be 1 or 2");
         ctx->adjoint_steps = value;
+    } else if (k == "tile_order") {
+        if (value < 0 || value > 64) return fail(SAM2D_ERR_PARAM, "tile_order must be 0..64");
+        ctx->tile_order = value;
     } else if (k == "adjoint_variant") {
         if (value >= num_variants()) return fail(SAM2D_ERR_PARAM, "variant %d out of range", value);
         ctx->adjoint_variant = value < 0 ? -1 : variant_single_step(value);
     }
+    else return fail(SAM2D_ERR_PARAM, "unknown option '%s'", key);
+    return SAM2D_OK;
+}
+
+int sam2d_get_option(sam2d_ctx *ctx, const char *key, int *value)
+{
+    clear_error();
+    if (!ctx || !key || !value) return fail(SAM2D_ERR_PARAM, "NULL argument");
+    std::string k(key);
+    if (k == "graph") *value = ctx->use_graph ? 1 : 0;
+    else if (k == "variant") *value = ctx->variant;
+    else if (k == "reset_quirk") *value = ctx->reset_quirk ? 1 : 0;
+    else if (k == "max_batch") *value = ctx->max_batch;
+    else if (k == "grid_cap") *value = ctx->grid_cap;
+    else if (k == "hbm_budget_gb") *value = ctx->hbm_budget_gb;
+    else if (k == "adjoint_steps") *value = ctx->adjoint_steps;
+    else if (k == "adjoint_variant") *value = ctx->adjoint_variant;
+    else if (k == "tile_order") *value = ctx->tile_order;
     else return fail(SAM2D_ERR_PARAM, "unknown option '%s'", key);
     return SAM2D_OK;
 }
@@ -733,6 +756,7 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
     double best_ms = 1e300;
     int best = default_variant_index();
     ctx->tune_ms.assign(num_variants(), 0.0);
+    ctx->tile_order = 0;                       // variants are compared in the plain order
     for (int v = 0; v < num_variants(); v++) {
         int waves, chunk;
         variant_shape(v, &waves, &chunk);
@@ -747,6 +771,33 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->tune_ms[v] = ms / steps;
         if (ms < best_ms) { best_ms = ms; best = v; }
+    }
+    // Second pass for a multi-step winner: the chunk-column order (stride 1
+    // = in order, 2..9), each timed twice over four times the steps (the
+    // strides differ by a few per cent), the faster kept.
+    if (variant_steps(best) > 1) {
+        int ntiles = 0;
+        SAM2D_TRY(stage_batch(ctx, plans, best, &ntiles));
+        const int osteps = 4 * steps;
+        double best_o = 1e300;
+        int order = 0;
+        for (int xo = 1; xo <= 9; xo++) {
+            ctx->tile_order = xo;
+            double t = 1e300;
+            for (int rep = 0; rep < 2; rep++) {
+                SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, best, 0, 2, 0, 0, nullptr));   // warm
+                HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+                SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, best, 2, 2 + osteps, 0, 0, nullptr));
+                HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+                HIP_TRY(hipEventSynchronize(ctx->ev1));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                t = std::min(t, (double)ms);
+            }
+            if (t < best_o) { best_o = t; order = xo; }
+        }
+        ctx->tile_order = order;
+        ctx->tune_ms[best] = best_o / osteps;
     }
     ctx->tuned_variant = best;
     if (best_variant) *best_variant = best;

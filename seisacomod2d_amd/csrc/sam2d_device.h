@@ -123,17 +123,25 @@ int variant_index(int waves, int chunk, int pf);
 // band_samples[j] from slice j of parity band_w ^ 1 and state_sample from the
 // in_slot state (each skipped when < 0).  kz_alias: every staged shot's KZ is
 // its KX (dt/dz == dt/dx).
+// Order of the chunk columns within each XCD's range of tiles (multi-step
+// kernels): xorder > 1 visits them with stride xorder, otherwise in order.
+// ngroups: z-groups per chunk column, the same for every staged shot.
+struct TileOrder {
+    int ngroups;
+    int xorder;
+};
 hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
                         bool kz_alias, int step, int in_slot, int out_slot, int band_mask,
                         int band_w, const int *band_samples, int state_sample, int max_ng,
-                        hipStream_t stream);
+                        TileOrder order, hipStream_t stream);
 // The recomputation form: the states after all three steps of a three-step
 // variant are stored, into slots in_slot+1 .. in_slot+3 (no receivers).  The
 // intermediate slots in_slot+1 and in_slot+2 hold p = px + pz in the px field
 // and no pz (a "p-sum" slot); in_slot+3 is a full state.
 bool ring_supported(int variant);
 hipError_t launch_stepn_ring(int variant, const ShotDev *d_shots, int nshots, int ntiles,
-                             int grid_cap, bool kz_alias, int step, int in_slot, hipStream_t stream);
+                             int grid_cap, bool kz_alias, int step, int in_slot, TileOrder order,
+                             hipStream_t stream);
 // The deferred gathers as a launch of their own (after the last multi-step
 // launch): band_samples[j] from slice j of parity band_r, the state slot.
 hipError_t launch_gather(const ShotDev *d_shots, int nshots, int max_ng, int band_r,

@@ -74,11 +74,11 @@ struct ShotPlan {
 };
 
 struct GraphKey {
-    int nshots, ntiles, variant, max_ng, grid_cap;
+    int nshots, ntiles, variant, max_ng, grid_cap, tile_order;
     bool operator<(const GraphKey &o) const
     {
-        return std::tie(nshots, ntiles, variant, max_ng, grid_cap) <
-               std::tie(o.nshots, o.ntiles, o.variant, o.max_ng, o.grid_cap);
+        return std::tie(nshots, ntiles, variant, max_ng, grid_cap, tile_order) <
+               std::tie(o.nshots, o.ntiles, o.variant, o.max_ng, o.grid_cap, o.tile_order);
     }
 };
 
@@ -109,7 +109,8 @@ struct sam2d_ctx {
     int grid_cap = 0;
     int hbm_budget_gb = 64;                // device memory a gradient batch may use
     int adjoint_steps = 2;                 // backward steps per adjoint launch (1 or 2)
-    int adjoint_variant = -1;              // single-step variant for adjoint tiles, -1 = follow
+    int adjoint_variant = -1;              // single-step variant for adjoint tiles, -1 = auto
+    int tile_order = 0;                    // chunk-column stride of multi-step kernels (tuned)
     std::vector<double> tune_ms;           // per variant, from the autotuner
     sam2d_stats stats = sam2d_stats();
     sam2d::AdjointState *adj = nullptr;
@@ -126,8 +127,10 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
 int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int n0, int n1, int max_ng,
                   int start_slot, int *end_slot);
 // The same on an explicit device descriptor array (tile shapes of `variant`).
-int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int nshots, int ntiles, int variant,
-                     int n0, int n1, int max_ng, int start_slot, int *end_slot);
+// desc: device descriptors of the staged shots with variant's tile shapes,
+// ngroups z-groups per chunk column (the same for every shot).
+int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int ngroups, int nshots, int ntiles,
+                     int variant, int n0, int n1, int max_ng, int start_slot, int *end_slot);
 void tile_shape(int nnz, int nnx, int waves, int chunk, int own_rows, int &ngroups, int &nchunks);
 }  // namespace sam2d
 

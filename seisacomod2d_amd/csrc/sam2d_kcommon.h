@@ -71,6 +71,28 @@ __device__ __forceinline__ int xcd_tile(int v, int per_xcd)
     return (v & 7) * per_xcd + (v >> 3);
 }
 
+// The same with the chunk columns of an XCD's range visited in a strided
+// order (xorder > 1; per_xcd a multiple of ngroups, tiles ordered z-group
+// fastest): the range's ncx columns go as 0, s, 2s, ..., 1, 1+s, ... with
+// s = xorder, each column's ngroups tiles in order.  Concurrent tiles are
+// spread along x, which measures up to 6 % faster for the multi-step kernel
+// on MI355X; the best stride depends on the tile shape and is autotuned.
+__device__ __forceinline__ int xcd_tile_ordered(int v, int per_xcd, int ngroups, int xorder)
+{
+    const int x = v & 7, p = v >> 3;
+    if (xorder <= 1) return x * per_xcd + p;
+    const int ncx = per_xcd / ngroups;
+    int q = p / ngroups;
+    const int g = p - q * ngroups;
+    int col = q;
+    for (int r = 0; r < xorder; r++) {
+        const int cnt = (ncx - r + xorder - 1) / xorder;   // columns r, r+s, ... < ncx
+        if (q < cnt) { col = r + q * xorder; break; }
+        q -= cnt;
+    }
+    return x * per_xcd + col * ngroups + g;
+}
+
 // Band buffer of the receiver rows: [2 parities][BAND_SLICES][band_h][nnx].
 // Slice j of parity w holds the pressure after the (j+1)-th step of a
 // multi-step launch; the pointer is offset so that it is indexed by the

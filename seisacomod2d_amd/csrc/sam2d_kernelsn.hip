@@ -56,7 +56,7 @@ template <int T, int WAVES, int PF, bool KZA, bool RING>
 __global__ __launch_bounds__(WAVES * 64)
 void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_xcd, int ntiles,
              int step, int in_slot, int out_slot, int band_mask, int band_w, int4 band_s,
-             int state_sample)
+             int state_sample, int ngroups, int xorder)
 {
     static_assert(T >= 2 && T <= BAND_SLICES + 1, "steps per pass");
     static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
@@ -72,7 +72,7 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                         blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 
     for (int v = blockIdx.x; v < vtotal; v += gridDim.x) {
-        const int tile = xcd_tile(v, per_xcd);
+        const int tile = xcd_tile_ordered(v, per_xcd, ngroups, xorder);
         if (tile >= ntiles) continue;
         const int s = shot_of_tile(shots, nshots, tile);
         const ShotDev &d = shots[s];
@@ -343,14 +343,25 @@ struct StepNArgs {
     const ShotDev *d;
     int ns, vt, per, nt, step, in, out, bmask, bw;
     int4 bs;
-    int ss;
+    int ss, ng, xo;
 };
+
+// Blocks per XCD range: a multiple of ngroups when the columns are reordered
+// (the tiles must then be whole columns of ngroups); otherwise the order is
+// the plain one.
+static int per_xcd_for(int ntiles, TileOrder &order)
+{
+    if (order.xorder > 1 && order.ngroups > 0 && ntiles % order.ngroups == 0)
+        return (ntiles / order.ngroups + 7) / 8 * order.ngroups;
+    order.xorder = 0;
+    return (ntiles + 7) / 8;
+}
 
 template <int T, int W, int PF, bool KZA, bool RING>
 static void launchn_t(dim3 g, hipStream_t s, const StepNArgs &a)
 {
     hipLaunchKernelGGL((k_stepn<T, W, PF, KZA, RING>), g, dim3(64 * W), 0, s, a.d, a.ns, a.vt,
-                       a.per, a.nt, a.step, a.in, a.out, a.bmask, a.bw, a.bs, a.ss);
+                       a.per, a.nt, a.step, a.in, a.out, a.bmask, a.bw, a.bs, a.ss, a.ng, a.xo);
 }
 
 template <int T, int W, bool RING>
@@ -379,12 +390,12 @@ static hipError_t launchn_waves(int waves, int pf, bool kza, dim3 g, hipStream_t
 hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
                         bool kz_alias, int step, int in_slot, int out_slot, int band_mask,
                         int band_w, const int *band_samples, int state_sample, int max_ng,
-                        hipStream_t stream)
+                        TileOrder order, hipStream_t stream)
 {
     int waves, chunk, pf;
     variant_shape_pf(variant, &waves, &chunk, &pf);
     const int T = variant_steps(variant);
-    const int per = (ntiles + 7) / 8;
+    const int per = per_xcd_for(ntiles, order);
     const int vtotal = per * 8;
     int grid = vtotal;
     if (grid_cap > 0 && grid > grid_cap) grid = (grid_cap + 7) / 8 * 8;
@@ -397,7 +408,7 @@ hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int nti
         state_sample = -1;
     }
     const StepNArgs a{d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot,
-                      band_mask, band_w, bs, state_sample};
+                      band_mask, band_w, bs, state_sample, order.ngroups, order.xorder};
     hipError_t e;
     switch (T) {
     case 2: e = launchn_waves<2>(waves, pf, kz_alias, g, stream, a); break;
@@ -413,17 +424,18 @@ hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int nti
 bool ring_supported(int variant) { return variant_steps(variant) == 3; }
 
 hipError_t launch_stepn_ring(int variant, const ShotDev *d_shots, int nshots, int ntiles,
-                             int grid_cap, bool kz_alias, int step, int in_slot, hipStream_t stream)
+                             int grid_cap, bool kz_alias, int step, int in_slot, TileOrder order,
+                             hipStream_t stream)
 {
     if (!ring_supported(variant)) return hipErrorInvalidValue;
     int waves, chunk, pf;
     variant_shape_pf(variant, &waves, &chunk, &pf);
-    const int per = (ntiles + 7) / 8;
+    const int per = per_xcd_for(ntiles, order);
     const int vtotal = per * 8;
     int grid = vtotal;
     if (grid_cap > 0 && grid > grid_cap) grid = (grid_cap + 7) / 8 * 8;
     const StepNArgs a{d_shots, nshots, vtotal, per, ntiles, step, in_slot, in_slot + 3,
-                      0, 0, make_int4(-1, -1, -1, -1), -1};
+                      0, 0, make_int4(-1, -1, -1, -1), -1, order.ngroups, order.xorder};
     const hipError_t e = launchn_waves<3, true>(waves, pf, kz_alias, dim3(grid), stream, a);
     if (e != hipSuccess) return e;
     return hipGetLastError();

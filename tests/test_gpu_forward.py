@@ -222,3 +222,30 @@ def test_multistep_batched_windows_bitwise():
             out = m.forward_batched(shots, wins)
             for a, b in zip(ref, out):
                 assert np.array_equal(a.view(np.int32), b.view(np.int32)), name
+
+
+def test_tile_order_bitwise():
+    # The chunk-column order of the multi-step kernels (set by the autotuner)
+    # permutes independent tiles: the same bits for every stride, with one
+    # shot, with batched windows and through graph replays.
+    g, shots, wins = _windows_case()
+    names = sam.variants()
+    with sam.Modeller(_grid(g, npml=24), g["vp"], g["rho"]) as m:
+        m.set_option("variant", 11)
+        ref = m.forward_windows(shots, wins)
+        ref1 = m.forward(shots[:1])
+        for name in ("threestep_waves1_chunk32", "twostep_waves2_chunk32_pf4",
+                     "fivestep_waves1_chunk32"):
+            m.set_option("variant", names.index(name))
+            for xo in (1, 2, 3, 4, 7, 64):
+                m.set_option("tile_order", xo)
+                out = m.forward_batched(shots, wins)
+                for a, b in zip(ref, out):
+                    assert np.array_equal(a.view(np.int32), b.view(np.int32)), (name, xo)
+                one = m.forward(shots[:1])
+                assert np.array_equal(one[0].view(np.int32), ref1[0].view(np.int32)), (name, xo)
+        m.set_option("tile_order", 0)
+        best = m.autotune(windows=wins, nshots=len(wins), steps=3)
+        out = m.forward_batched(shots, wins)
+        for a, b in zip(ref, out):
+            assert np.array_equal(a.view(np.int32), b.view(np.int32)), names[best]

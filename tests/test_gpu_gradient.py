@@ -115,12 +115,15 @@ def test_multistep_forward_sweep_bitwise():
                 for a, b in zip(g[2], ref[2]):
                     assert np.array_equal(a, b), (name, k)
         # The three-step recomputation stores p = px + pz for its intermediate
-        # states; both adjoint kernels read those slots.
+        # states; both adjoint kernels read those slots.  The chunk-column
+        # order of the forward sweep and the recomputation changes nothing.
         m.set_option("variant", names.index("threestep_waves2_chunk64"))
-        m.set_option("adjoint_steps", 1)
-        for k in (12, 8, 7):
-            g = m.gradient(shots, adj, ckpt_k=k)
-            assert np.array_equal(g[0], ref[0]) and np.array_equal(g[1], ref[1]), k
+        for steps, xo in ((1, 0), (2, 3), (1, 5)):
+            m.set_option("adjoint_steps", steps)
+            m.set_option("tile_order", xo)
+            for k in (12, 8, 7):
+                g = m.gradient(shots, adj, ckpt_k=k)
+                assert np.array_equal(g[0], ref[0]) and np.array_equal(g[1], ref[1]), (steps, xo, k)
 
 
 def _deep_setup(dz=10.0, T=0.16):
