@@ -154,6 +154,9 @@ def copy_probe(nbytes=1 << 30):
 
 # --------------------------------------------------------------- config 1
 def bench_plain(args, sam):
+    if not str(args.variant).lstrip("-").isdigit():
+        args.variant = sam.variants().index(args.variant)
+    args.variant = int(args.variant)
     vp, rho = model()
     t_rec = T_REC if args.profile_steps <= 0 else args.profile_steps * DT
     grid = sam.Grid(NX, NZ, H, H, t_rec, DT, DT, FM, npml=NPML)
@@ -412,7 +415,8 @@ def main():
     ap.add_argument("--steps", type=int, default=8, help="timed shots")
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--variant", default="-1",
+                    help="kernel variant: index or name (-1: autotuned)")
     ap.add_argument("--tile-order", type=int, default=-1,
                     help="chunk-column stride of the multi-step kernels (-1: autotuned, or 0 "
                          "with --variant)")

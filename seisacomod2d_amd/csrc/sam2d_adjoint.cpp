@@ -239,7 +239,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     // Two backward steps per pass need every residual in a band (nothing to
     // inject between the steps) and strips of OWN_Z2 rows.
     const bool two_step = ctx->adjoint_steps == 2 && all_band;
-    int ntiles2 = 0;
+    int ntiles2 = 0, groups2 = 0;
     if (two_step) {
         int w2, c2;
         variant_shape(variant, &w2, &c2);
@@ -250,6 +250,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
             dd.tile_begin = ntiles2;
             ntiles2 += dd.ngroups * dd.nchunks;
         }
+        groups2 = desc2[0].ngroups;
         if (ns > A.cap_desc2) {
             dev_free(A.d_desc2); A.d_desc2 = nullptr; A.cap_desc2 = 0;
             SAM2D_TRY(dev_alloc((void **)&A.d_desc2, sizeof(AdjDev) * ns));
@@ -344,6 +345,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
                 HIP_TRY(launch_adjoint_step2(variant, A.d_desc2, ns, ntiles2, kza, n_top, lam,
                                              lam ^ 1, fb - 1, fb, fc, rec_sample(ta, n_top),
                                              rec_sample(ta, n_top - 1), psum(fc) | psum(fb) << 1,
+                                             TileOrder{groups2, ctx->adjoint_tile_order},
                                              ctx->stream));
                 lam ^= 1;
             }
@@ -352,6 +354,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
                 HIP_TRY(launch_inject(A.d_desc, ns, max_cells, lam, rec_sample(ta, n), ctx->stream));
             HIP_TRY(launch_adjoint_step(variant, A.d_desc, ns, ntiles, kza, n, lam, lam ^ 1, n - n0,
                                         n - n0 + 1, rec_sample(ta, n), psum(n - n0 + 1),
+                                        TileOrder{desc[0].ngroups, ctx->adjoint_tile_order},
                                         ctx->stream));
             lam ^= 1;
         }

@@ -116,7 +116,7 @@ template <int WAVES, int PF, bool KZA>
 __global__ __launch_bounds__(WAVES * 64)
 void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
                int ntiles, int step, int lam_in, int lam_out, int fwd_prev, int fwd_cur,
-               int rec_k, int psum)
+               int rec_k, int psum, int ngroups, int xorder)
 {
     static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
     using gcchar = const SAM2D_GLOBAL char;
@@ -125,7 +125,7 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
     for (int v = blockIdx.x; v < vtotal; v += gridDim.x) {
-        const int tile = xcd_tile(v, per_xcd);
+        const int tile = xcd_tile_ordered(v, per_xcd, ngroups, xorder);
         if (tile >= ntiles) continue;
         const int s = shot_of_tile(shots, nshots, tile);
         const AdjDev &d = shots[s];
@@ -396,7 +396,7 @@ template <int WAVES, int PF, bool KZA>
 __global__ __launch_bounds__(WAVES * 64)
 void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
                 int ntiles, int step, int lam_in, int lam_out, int fa, int fb, int fc,
-                int rec1, int rec2, int psum)
+                int rec1, int rec2, int psum, int ngroups, int xorder)
 {
     static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
     using gcchar = const SAM2D_GLOBAL char;
@@ -405,7 +405,7 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
     for (int v = blockIdx.x; v < vtotal; v += gridDim.x) {
-        const int tile = xcd_tile(v, per_xcd);
+        const int tile = xcd_tile_ordered(v, per_xcd, ngroups, xorder);
         if (tile >= ntiles) continue;
         const int s = shot_of_tile(shots, nshots, tile);
         const AdjDev &d = shots[s];
@@ -754,37 +754,38 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
 
 template <int W, int PF, bool KZA>
 static void launch_adj_t(dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt, int per, int nt,
-                         int step, int li, int lo, int fp, int fc, int rk, int ps)
+                         int step, int li, int lo, int fp, int fc, int rk, int ps, TileOrder o)
 {
     hipLaunchKernelGGL((k_adjoint<W, PF, KZA>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step, li,
-                       lo, fp, fc, rk, ps);
+                       lo, fp, fc, rk, ps, o.ngroups, o.xorder);
 }
 
 template <int W>
 static void launch_adj_w(int pf, bool kza, dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt,
                          int per, int nt, int step, int li, int lo, int fp, int fc, int rk,
-                         int ps)
+                         int ps, TileOrder o)
 {
-    if (pf == 2 && kza) launch_adj_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps);
-    else if (pf == 2) launch_adj_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps);
-    else if (kza) launch_adj_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps);
-    else launch_adj_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps);
+    if (pf == 2 && kza) launch_adj_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps, o);
+    else if (pf == 2) launch_adj_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps, o);
+    else if (kza) launch_adj_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps, o);
+    else launch_adj_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps, o);
 }
 
 hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
                                bool kz_alias, int step, int lam_in, int lam_out, int fwd_prev,
-                               int fwd_cur, int rec_k, int psum, hipStream_t stream)
+                               int fwd_cur, int rec_k, int psum, TileOrder order,
+                               hipStream_t stream)
 {
     int waves, chunk, pf;
     variant_shape_pf(variant, &waves, &chunk, &pf);
-    const int per = (ntiles + 7) / 8;
+    const int per = per_xcd_for(ntiles, order);
     const int vtotal = per * 8;
     const dim3 g(vtotal);
     switch (waves) {
-    case 1: launch_adj_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum); break;
-    case 2: launch_adj_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum); break;
-    case 4: launch_adj_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum); break;
-    case 8: launch_adj_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum); break;
+    case 1: launch_adj_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum, order); break;
+    case 2: launch_adj_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum, order); break;
+    case 4: launch_adj_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum, order); break;
+    case 8: launch_adj_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum, order); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -793,37 +794,38 @@ hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int
 template <int W, int PF, bool KZA>
 static void launch_adj2_t(dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt, int per,
                           int nt, int step, int li, int lo, int fa, int fb, int fc, int r1, int r2,
-                          int ps)
+                          int ps, TileOrder o)
 {
     hipLaunchKernelGGL((k_adjoint2<W, PF, KZA>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step,
-                       li, lo, fa, fb, fc, r1, r2, ps);
+                       li, lo, fa, fb, fc, r1, r2, ps, o.ngroups, o.xorder);
 }
 
 template <int W>
 static void launch_adj2_w(int pf, bool kza, dim3 g, hipStream_t s, const AdjDev *d, int ns,
                           int vt, int per, int nt, int step, int li, int lo, int fa, int fb, int fc,
-                          int r1, int r2, int ps)
+                          int r1, int r2, int ps, TileOrder o)
 {
-    if (pf == 2 && kza) launch_adj2_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps);
-    else if (pf == 2) launch_adj2_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps);
-    else if (kza) launch_adj2_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps);
-    else launch_adj2_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps);
+    if (pf == 2 && kza) launch_adj2_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps, o);
+    else if (pf == 2) launch_adj2_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps, o);
+    else if (kza) launch_adj2_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps, o);
+    else launch_adj2_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps, o);
 }
 
 hipError_t launch_adjoint_step2(int variant, const AdjDev *d_adj, int nshots, int ntiles,
                                 bool kz_alias, int step, int lam_in, int lam_out, int fa, int fb,
-                                int fc, int rec1, int rec2, int psum, hipStream_t stream)
+                                int fc, int rec1, int rec2, int psum, TileOrder order,
+                                hipStream_t stream)
 {
     int waves, chunk, pf;
     variant_shape_pf(variant, &waves, &chunk, &pf);
-    const int per = (ntiles + 7) / 8;
+    const int per = per_xcd_for(ntiles, order);
     const int vtotal = per * 8;
     const dim3 g(vtotal);
     switch (waves) {
-    case 1: launch_adj2_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum); break;
-    case 2: launch_adj2_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum); break;
-    case 4: launch_adj2_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum); break;
-    case 8: launch_adj2_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum); break;
+    case 1: launch_adj2_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum, order); break;
+    case 2: launch_adj2_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum, order); break;
+    case 4: launch_adj2_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum, order); break;
+    case 8: launch_adj2_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum, order); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

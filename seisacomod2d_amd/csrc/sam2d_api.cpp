@@ -706,6 +706,9 @@ int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value)
     } else if (k == "tile_order") {
         if (value < 0 || value > 64) return fail(SAM2D_ERR_PARAM, "tile_order must be 0..64");
         ctx->tile_order = value;
+    } else if (k == "adjoint_tile_order") {
+        if (value < 0 || value > 64) return fail(SAM2D_ERR_PARAM, "adjoint_tile_order must be 0..64");
+        ctx->adjoint_tile_order = value;
     } else if (k == "adjoint_variant") {
         if (value >= num_variants()) return fail(SAM2D_ERR_PARAM, "variant %d out of range", value);
         ctx->adjoint_variant = value < 0 ? -1 : variant_single_step(value);
@@ -728,6 +731,7 @@ int sam2d_get_option(sam2d_ctx *ctx, const char *key, int *value)
     else if (k == "adjoint_steps") *value = ctx->adjoint_steps;
     else if (k == "adjoint_variant") *value = ctx->adjoint_variant;
     else if (k == "tile_order") *value = ctx->tile_order;
+    else if (k == "adjoint_tile_order") *value = ctx->adjoint_tile_order;
     else return fail(SAM2D_ERR_PARAM, "unknown option '%s'", key);
     return SAM2D_OK;
 }

@@ -76,6 +76,23 @@ struct AdjDev {
                            // (-0 elsewhere), or NULL: injected by k_inject
 };
 
+// Order of the chunk columns within each XCD's range of tiles (multi-step
+// kernels): xorder > 1 visits them with stride xorder, otherwise in order.
+// ngroups: z-groups per chunk column, the same for every staged shot.
+struct TileOrder {
+    int ngroups;
+    int xorder;
+};
+// Blocks per XCD range for a launch of ntiles tiles: a multiple of ngroups
+// when the columns are reordered (the tiles must then be whole columns of
+// ngroups); otherwise the order falls back to the plain one.
+inline int per_xcd_for(int ntiles, TileOrder &order)
+{
+    if (order.xorder > 1 && order.ngroups > 0 && ntiles % order.ngroups == 0)
+        return (ntiles / order.ngroups + 7) / 8 * order.ngroups;
+    order.xorder = 0;
+    return (ntiles + 7) / 8;
+}
 // One backward step n of every staged shot: adjoint state lam_in -> lam_out,
 // forward states S(n-1) in ring slot fwd_prev and S(n) in fwd_cur; rec_k >= 0
 // injects residual sample rec_k of the shots that carry an rband.
@@ -83,7 +100,8 @@ struct AdjDev {
 // slot (launch_stepn_ring).
 hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
                                bool kz_alias, int step, int lam_in, int lam_out, int fwd_prev,
-                               int fwd_cur, int rec_k, int psum, hipStream_t stream);
+                               int fwd_cur, int rec_k, int psum, TileOrder order,
+                               hipStream_t stream);
 // Backward steps n and n-1 in one pass (all shots must carry an rband or no
 // receivers): forward vx, vz after step n-2 in ring slot fa, after n-1 in fb;
 // px, pz after n-1 in fb, after n in fc; residual samples rec1 (step n) and
@@ -91,7 +109,8 @@ hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int
 // OWN_Z2-row strips.
 hipError_t launch_adjoint_step2(int variant, const AdjDev *d_adj, int nshots, int ntiles,
                                 bool kz_alias, int step, int lam_in, int lam_out, int fa, int fb,
-                                int fc, int rec1, int rec2, int psum, hipStream_t stream);
+                                int fc, int rec1, int rec2, int psum, TileOrder order,
+                                hipStream_t stream);
 // Dense residual band [real_nt][band_cells]: element (k, i) is the sum, in
 // rid order, of res[rid[e] * real_nt + k] over e in [rptr[u], rptr[u+1]) with
 // u = bmap[i], starting from -0; -0 where bmap[i] < 0.
@@ -123,13 +142,6 @@ int variant_index(int waves, int chunk, int pf);
 // band_samples[j] from slice j of parity band_w ^ 1 and state_sample from the
 // in_slot state (each skipped when < 0).  kz_alias: every staged shot's KZ is
 // its KX (dt/dz == dt/dx).
-// Order of the chunk columns within each XCD's range of tiles (multi-step
-// kernels): xorder > 1 visits them with stride xorder, otherwise in order.
-// ngroups: z-groups per chunk column, the same for every staged shot.
-struct TileOrder {
-    int ngroups;
-    int xorder;
-};
 hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
                         bool kz_alias, int step, int in_slot, int out_slot, int band_mask,
                         int band_w, const int *band_samples, int state_sample, int max_ng,

@@ -111,6 +111,7 @@ struct sam2d_ctx {
     int adjoint_steps = 2;                 // backward steps per adjoint launch (1 or 2)
     int adjoint_variant = -1;              // single-step variant for adjoint tiles, -1 = auto
     int tile_order = 0;                    // chunk-column stride of multi-step kernels (tuned)
+    int adjoint_tile_order = 7;            // chunk-column stride of the adjoint kernels
     std::vector<double> tune_ms;           // per variant, from the autotuner
     sam2d_stats stats = sam2d_stats();
     sam2d::AdjointState *adj = nullptr;

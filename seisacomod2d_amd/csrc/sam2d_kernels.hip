@@ -241,7 +241,7 @@ static const int kChunks[] = {16, 32, 64, 128, 256};
 static const int kPF[] = {2, 4};
 static const int kNTS[] = {0, 1};
 static const int nW = 4, nC = 5, nP = 2, nN = 2;
-static const int kChunks2[] = {32, 64, 128, 256, 512};
+static const int kChunks2[] = {16, 32, 64, 128, 256};
 static const int nSingle = nW * nC * nP * nN;     // single-step variants first
 static const int nMultiEach = nW * 5 * nP;        // then 2-, 3-, 4- and 5-step variants
 static const int kMaxSteps = 5;

@@ -346,17 +346,6 @@ struct StepNArgs {
     int ss, ng, xo;
 };
 
-// Blocks per XCD range: a multiple of ngroups when the columns are reordered
-// (the tiles must then be whole columns of ngroups); otherwise the order is
-// the plain one.
-static int per_xcd_for(int ntiles, TileOrder &order)
-{
-    if (order.xorder > 1 && order.ngroups > 0 && ntiles % order.ngroups == 0)
-        return (ntiles / order.ngroups + 7) / 8 * order.ngroups;
-    order.xorder = 0;
-    return (ntiles + 7) / 8;
-}
-
 template <int T, int W, int PF, bool KZA, bool RING>
 static void launchn_t(dim3 g, hipStream_t s, const StepNArgs &a)
 {

@@ -165,16 +165,23 @@ def test_two_step_adjoint_bitwise_equal_one_step(dz):
     names = sam.variants()
     with sam.Modeller(grid, vp, rho) as m:
         adj = [rng.standard_normal((s.nrec, m.real_nt)).astype(np.float32) for s in shots]
-        m.set_option("variant", names.index("waves2_chunk32_pf2"))
+        m.set_option("adjoint_variant", names.index("waves2_chunk32_pf2"))
         m.set_option("adjoint_steps", 1)
         ref = m.gradient(shots, adj, ckpt_k=13)
         m.set_option("adjoint_steps", 2)
         for name in ("waves1_chunk16_pf2", "waves2_chunk32_pf2", "waves4_chunk64_pf4",
                      "waves8_chunk128_pf2_nts", "waves1_chunk256_pf4"):
-            m.set_option("variant", names.index(name))
+            m.set_option("adjoint_variant", names.index(name))
             for k in (13, 6, 1):
                 g = m.gradient(shots, adj, ckpt_k=k)
                 assert np.array_equal(g[0], ref[0]) and np.array_equal(g[1], ref[1]), (name, k)
+        # the chunk-column order of the adjoint tiles permutes independent tiles
+        for xo in (2, 3, 5):
+            m.set_option("adjoint_tile_order", xo)
+            for steps in (1, 2):
+                m.set_option("adjoint_steps", steps)
+                g = m.gradient(shots, adj, ckpt_k=13)
+                assert np.array_equal(g[0], ref[0]) and np.array_equal(g[1], ref[1]), (xo, steps)
 
 
 def test_gradient_forward_seismograms_match_forward():
