@@ -156,7 +156,8 @@ int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
  * the single-step form of the given variant index; -1 = automatic, the
  * default), "tile_order" (0 or 1 = chunk columns of each XCD's tile range in
  * order, s = 2..64 = with stride s, for the multi-step kernels; set by
- * sam2d_autotune, default 0; results are the same bits). */
+ * sam2d_autotune, default 0; results are the same bits),
+ * "adjoint_tile_order" (the same for the adjoint kernels, default 7). */
 int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value);
 /* Current value of an option above (after sam2d_autotune: "tile_order"). */
 int sam2d_get_option(sam2d_ctx *ctx, const char *key, int *value);
