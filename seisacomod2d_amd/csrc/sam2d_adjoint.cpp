@@ -5,9 +5,11 @@
 //      enters step m*k into checkpoint m (and the seismograms if asked for);
 //   2. backward, segment by segment from the last: restore checkpoint m into
 //      ring slot 0, recompute steps [m*k, m*k + k) into ring slots 1..k
-//      (bit-identical to the first pass: same kernel, same inputs), then run
-//      the adjoint steps of the segment in reverse, each reading the forward
-//      states it needs from the ring;
+//      (bit-identical to the first pass: the same arithmetic per cell, three
+//      steps per launch where the forward variant allows), then run the
+//      adjoint steps of the segment in reverse, two per pass where every shot
+//      has a residual band, each reading the forward states it needs from the
+//      ring;
 //   3. copy dJ/dkappa, dJ/dbu2, dJ/dbu1 to the host and chain them in FP64 to
 //      dJ/dvp and dJ/drho through the buoyancy staggering, kappa = vp^2 rho and
 //      the sponge padding (the transpose of cpu_stagger,

@@ -1,6 +1,8 @@
-// sam2d_adjoint_kernels.hip -- gfx950 kernels of one backward (adjoint) step
+// sam2d_adjoint_kernels.hip -- gfx950 kernels of the backward (adjoint) steps
 // of the staggered-grid scheme in sam2d_kernels.hip, with the FWI gradient
-// accumulation fused in.
+// accumulation fused in: k_adjoint (one step per pass), k_adjoint2 (two
+// steps per pass, the same bits), the residual band builder k_band and the
+// fallback injection k_inject.
 //
 // The reference has no adjoint (job_t.adj is declared and unused,
 // src/src_cpu/modelling.h:51).  This is the exact transpose of the FP32
