@@ -168,12 +168,16 @@ def bench_plain(args, sam):
         if args.variant >= 0:
             m.set_option("variant", args.variant)
             m.set_option("tile_order", max(args.tile_order, 0))
+            m.set_option("chunk", max(args.chunk, 0))
         else:
             best = m.autotune(steps=6)
             if args.tile_order >= 0:
                 m.set_option("tile_order", args.tile_order)
+            if args.chunk >= 0:
+                m.set_option("chunk", args.chunk)
             print(f"# autotune picked {sam.variants()[best]}, tile order "
-                  f"{m.get_option('tile_order')}", file=sys.stderr, flush=True)
+                  f"{m.get_option('tile_order')}, chunk {m.get_option('chunk')}",
+                  file=sys.stderr, flush=True)
             if args.tuning:
                 for name, ms in zip(sam.variants(), m.tuning()):
                     print(f"#   {name:28s} {ms * 1e3:8.1f} us/step", file=sys.stderr)
@@ -195,6 +199,7 @@ def bench_plain(args, sam):
         wall = time.perf_counter() - t0
         variant = sam.variants()[st["variant"]]
         tile_order = m.get_option("tile_order")
+        chunk = m.get_option("chunk")
         spl = st["steps_per_launch"]
         used_graph = st["used_graph"]
         hip.free(seis)
@@ -236,6 +241,7 @@ def bench_plain(args, sam):
         "config": {"workload": "config1_plain_forward_5000x5000_nt4000_npml50",
                    "grid": [NX, NZ], "padded": [m.nnx, m.nnz], "nt": m.fd_nt, "receivers": ng,
                    "shots_per_job": NSHOTS, "kernel_variant": variant, "tile_order": tile_order,
+                   "chunk": chunk,
                    "hipgraph": used_graph,
                    "profile_run": args.profile_steps > 0},
         "roofline": roof,
@@ -417,6 +423,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", default="-1",
                     help="kernel variant: index or name (-1: autotuned)")
+    ap.add_argument("--chunk", type=int, default=-1,
+                    help="march length of the multi-step kernels (-1: autotuned, or the "
+                         "variant's with --variant; 0 = the variant's)")
     ap.add_argument("--tile-order", type=int, default=-1,
                     help="chunk-column stride of the multi-step kernels (-1: autotuned, or 0 "
                          "with --variant)")

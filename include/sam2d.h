@@ -157,7 +157,9 @@ int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
  * default), "tile_order" (0 or 1 = chunk columns of each XCD's tile range in
  * order, s = 2..64 = with stride s, for the multi-step kernels; set by
  * sam2d_autotune, default 0; results are the same bits),
- * "adjoint_tile_order" (the same for the adjoint kernels, default 7). */
+ * "adjoint_tile_order" (the same for the adjoint kernels, default 7),
+ * "chunk" (columns each multi-step tile marches, 0 = the variant's own; set
+ * by sam2d_autotune; results are the same bits). */
 int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value);
 /* Current value of an option above (after sam2d_autotune: "tile_order"). */
 int sam2d_get_option(sam2d_ctx *ctx, const char *key, int *value);
@@ -165,7 +167,8 @@ int sam2d_get_option(sam2d_ctx *ctx, const char *key, int *value);
 /* Project-written autotuner: times every kernel variant (z-strips per
  * workgroup x march length along x) for a few steps on this context's grid
  * (or on the given windows) and selects the fastest; for a multi-step winner
- * it then times the chunk-column strides 1..9 and sets "tile_order".
+ * it then times march lengths around the variant's jointly with the
+ * chunk-column strides 1..9 and sets "chunk" and "tile_order".
  * best_variant may be NULL. */
 int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps,
                    int *best_variant);

@@ -287,6 +287,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     if (variant_steps(fwd_variant) > 1) {
         int w2, c2;
         variant_shape(fwd_variant, &w2, &c2);
+        c2 = tile_chunk(ctx, fwd_variant);
         std::vector<ShotDev> d2(ctx->staged_desc.begin(), ctx->staged_desc.begin() + ns);
         int t2 = 0;
         for (auto &dd : d2) {

@@ -214,12 +214,20 @@ static void ones_range(const std::vector<float> &prof_x, int nnx, int &lo, int &
     while (hi < nnx - 1 && ones(hi + 1)) hi++;
 }
 
+int tile_chunk(const sam2d_ctx *ctx, int variant)
+{
+    int waves, chunk;
+    variant_shape(variant, &waves, &chunk);
+    return ctx->chunk > 0 && variant_steps(variant) > 1 ? ctx->chunk : chunk;
+}
+
 // Make `plans` resident: buffers, descriptors, zeroed state and seismograms.
 int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *ntiles_out,
                 int nslots)
 {
     int waves, chunk;
     variant_shape(variant, &waves, &chunk);
+    chunk = tile_chunk(ctx, variant);
     const int ns = (int)plans.size();
     if ((int)ctx->bufs.size() < ns) ctx->bufs.resize(ns);
     std::vector<ShotDev> desc(ns);
@@ -389,7 +397,8 @@ int run_time_loop(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int max_n
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     bool used_graph = false;
     if (ctx->use_graph) {
-        GraphKey key{nshots, ntiles, variant, max_ng, ctx->grid_cap, ctx->tile_order};
+        GraphKey key{nshots, ntiles, variant, max_ng, ctx->grid_cap, ctx->tile_order,
+                     tile_chunk(ctx, variant)};
         hipGraphExec_t exec = nullptr;
         auto it = ctx->graphs.find(key);
         if (it != ctx->graphs.end()) exec = it->second;
@@ -706,6 +715,10 @@ int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value)
     } else if (k == "tile_order") {
         if (value < 0 || value > 64) return fail(SAM2D_ERR_PARAM, "tile_order must be 0..64");
         ctx->tile_order = value;
+    } else if (k == "chunk") {
+        if (value != 0 && (value < 8 || value > 4096))
+            return fail(SAM2D_ERR_PARAM, "chunk must be 0 or 8..4096");
+        ctx->chunk = value;
     } else if (k == "adjoint_tile_order") {
         if (value < 0 || value > 64) return fail(SAM2D_ERR_PARAM, "adjoint_tile_order must be 0..64");
         ctx->adjoint_tile_order = value;
@@ -732,6 +745,7 @@ int sam2d_get_option(sam2d_ctx *ctx, const char *key, int *value)
     else if (k == "adjoint_variant") *value = ctx->adjoint_variant;
     else if (k == "tile_order") *value = ctx->tile_order;
     else if (k == "adjoint_tile_order") *value = ctx->adjoint_tile_order;
+    else if (k == "chunk") *value = ctx->chunk;
     else return fail(SAM2D_ERR_PARAM, "unknown option '%s'", key);
     return SAM2D_OK;
 }
@@ -761,6 +775,7 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
     int best = default_variant_index();
     ctx->tune_ms.assign(num_variants(), 0.0);
     ctx->tile_order = 0;                       // variants are compared in the plain order
+    ctx->chunk = 0;                            // and with their own march lengths
     for (int v = 0; v < num_variants(); v++) {
         int waves, chunk;
         variant_shape(v, &waves, &chunk);
@@ -776,31 +791,44 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
         ctx->tune_ms[v] = ms / steps;
         if (ms < best_ms) { best_ms = ms; best = v; }
     }
-    // Second pass for a multi-step winner: the chunk-column order (stride 1
-    // = in order, 2..9), each timed twice over four times the steps (the
-    // strides differ by a few per cent), the faster kept.
+    // Second pass for a multi-step winner: its march length (the variant's
+    // chunk scaled by 1/2 .. 3/2, multiples of 4) jointly with the
+    // chunk-column order (stride 1 = in order, 2..9).  Each pair is timed
+    // twice over four times the steps (they differ by a few per cent) and the
+    // fastest kept.
     if (variant_steps(best) > 1) {
-        int ntiles = 0;
-        SAM2D_TRY(stage_batch(ctx, plans, best, &ntiles));
+        int w0, c0;
+        variant_shape(best, &w0, &c0);
+        std::vector<int> chunks;
+        for (int eighths : {4, 5, 6, 7, 8, 9, 10, 12}) {
+            const int c = std::max(16, (c0 * eighths / 8 + 2) / 4 * 4);
+            if (std::find(chunks.begin(), chunks.end(), c) == chunks.end()) chunks.push_back(c);
+        }
         const int osteps = 4 * steps;
         double best_o = 1e300;
-        int order = 0;
-        for (int xo = 1; xo <= 9; xo++) {
-            ctx->tile_order = xo;
-            double t = 1e300;
-            for (int rep = 0; rep < 2; rep++) {
-                SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, best, 0, 2, 0, 0, nullptr));   // warm
-                HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-                SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, best, 2, 2 + osteps, 0, 0, nullptr));
-                HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-                HIP_TRY(hipEventSynchronize(ctx->ev1));
-                float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-                t = std::min(t, (double)ms);
+        int best_order = 0, best_chunk = 0;
+        for (int c : chunks) {
+            ctx->chunk = c == c0 ? 0 : c;
+            int ntiles = 0;
+            SAM2D_TRY(stage_batch(ctx, plans, best, &ntiles));
+            for (int xo = 1; xo <= 9; xo++) {
+                ctx->tile_order = xo;
+                double t = 1e300;
+                for (int rep = 0; rep < 2; rep++) {
+                    SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, best, 0, 2, 0, 0, nullptr));   // warm
+                    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+                    SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, best, 2, 2 + osteps, 0, 0, nullptr));
+                    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+                    HIP_TRY(hipEventSynchronize(ctx->ev1));
+                    float ms = 0;
+                    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                    t = std::min(t, (double)ms);
+                }
+                if (t < best_o) { best_o = t; best_order = xo; best_chunk = ctx->chunk; }
             }
-            if (t < best_o) { best_o = t; order = xo; }
         }
-        ctx->tile_order = order;
+        ctx->tile_order = best_order;
+        ctx->chunk = best_chunk;
         ctx->tune_ms[best] = best_o / osteps;
     }
     ctx->tuned_variant = best;

@@ -74,11 +74,12 @@ struct ShotPlan {
 };
 
 struct GraphKey {
-    int nshots, ntiles, variant, max_ng, grid_cap, tile_order;
+    int nshots, ntiles, variant, max_ng, grid_cap, tile_order, chunk;
     bool operator<(const GraphKey &o) const
     {
-        return std::tie(nshots, ntiles, variant, max_ng, grid_cap, tile_order) <
-               std::tie(o.nshots, o.ntiles, o.variant, o.max_ng, o.grid_cap, o.tile_order);
+        return std::tie(nshots, ntiles, variant, max_ng, grid_cap, tile_order, chunk) <
+               std::tie(o.nshots, o.ntiles, o.variant, o.max_ng, o.grid_cap, o.tile_order,
+                        o.chunk);
     }
 };
 
@@ -112,6 +113,7 @@ struct sam2d_ctx {
     int adjoint_variant = -1;              // single-step variant for adjoint tiles, -1 = auto
     int tile_order = 0;                    // chunk-column stride of multi-step kernels (tuned)
     int adjoint_tile_order = 7;            // chunk-column stride of the adjoint kernels
+    int chunk = 0;                         // march length of multi-step kernels, 0 = variant's
     std::vector<double> tune_ms;           // per variant, from the autotuner
     sam2d_stats stats = sam2d_stats();
     sam2d::AdjointState *adj = nullptr;
@@ -123,6 +125,9 @@ int plan_shot(sam2d_ctx *ctx, const sam2d_shot &sh, DeviceDomain *dom, ShotPlan 
 int make_plans(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots, const int *windows,
                std::vector<ShotPlan> &plans);
 int pick_variant(sam2d_ctx *ctx);
+// March length (columns per tile) of `variant`: the "chunk" option for a
+// multi-step variant when set, else the variant's own.
+int tile_chunk(const sam2d_ctx *ctx, int variant);
 int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *ntiles_out,
                 int nslots = 2);
 int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int n0, int n1, int max_ng,

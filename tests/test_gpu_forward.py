@@ -225,9 +225,10 @@ def test_multistep_batched_windows_bitwise():
 
 
 def test_tile_order_bitwise():
-    # The chunk-column order of the multi-step kernels (set by the autotuner)
-    # permutes independent tiles: the same bits for every stride, with one
-    # shot, with batched windows and through graph replays.
+    # The chunk-column order and the march length of the multi-step kernels
+    # (set by the autotuner) only regroup independent cells into tiles: the
+    # same bits for every stride and length, with one shot, with batched
+    # windows and through graph replays.
     g, shots, wins = _windows_case()
     names = sam.variants()
     with sam.Modeller(_grid(g, npml=24), g["vp"], g["rho"]) as m:
@@ -237,14 +238,16 @@ def test_tile_order_bitwise():
         for name in ("threestep_waves1_chunk32", "twostep_waves2_chunk32_pf4",
                      "fivestep_waves1_chunk32"):
             m.set_option("variant", names.index(name))
-            for xo in (1, 2, 3, 4, 7, 64):
+            for xo, chunk in ((1, 0), (2, 20), (3, 0), (4, 44), (7, 0), (64, 8)):
                 m.set_option("tile_order", xo)
+                m.set_option("chunk", chunk)
                 out = m.forward_batched(shots, wins)
                 for a, b in zip(ref, out):
                     assert np.array_equal(a.view(np.int32), b.view(np.int32)), (name, xo)
                 one = m.forward(shots[:1])
                 assert np.array_equal(one[0].view(np.int32), ref1[0].view(np.int32)), (name, xo)
         m.set_option("tile_order", 0)
+        m.set_option("chunk", 0)
         best = m.autotune(windows=wins, nshots=len(wins), steps=3)
         out = m.forward_batched(shots, wins)
         for a, b in zip(ref, out):

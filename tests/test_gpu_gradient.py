@@ -118,9 +118,10 @@ def test_multistep_forward_sweep_bitwise():
         # states; both adjoint kernels read those slots.  The chunk-column
         # order of the forward sweep and the recomputation changes nothing.
         m.set_option("variant", names.index("threestep_waves2_chunk64"))
-        for steps, xo in ((1, 0), (2, 3), (1, 5)):
+        for steps, xo, chunk in ((1, 0, 0), (2, 3, 40), (1, 5, 24)):
             m.set_option("adjoint_steps", steps)
             m.set_option("tile_order", xo)
+            m.set_option("chunk", chunk)
             for k in (12, 8, 7):
                 g = m.gradient(shots, adj, ckpt_k=k)
                 assert np.array_equal(g[0], ref[0]) and np.array_equal(g[1], ref[1]), (steps, xo, k)
