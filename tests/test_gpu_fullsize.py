@@ -1,0 +1,70 @@
+"""Properties at the benchmark configurations' full sizes (GPU).
+
+The FP64 oracle cannot run a full 5000 x 5000 x 4000-step shot or a 3201 x
+1201 x 8000-step gradient in test time, so at these sizes the tests check
+what holds independently of size: the tuned multi-step path gives the same
+bits as the single-step kernel (which the oracle pins at smaller sizes and on
+config 1's full grid over its first steps), every trace is finite and carries
+the direct wave, and the gradient is the same bits for any checkpoint
+interval and for the one- and two-step adjoints.
+"""
+import numpy as np
+import pytest
+
+import seisacomod2d_amd as sam
+
+pytestmark = pytest.mark.gpu
+
+
+def test_config1_full_record_tuned_equals_single_step():
+    nx = nz = 5000
+    vp = np.full((nx, nz), 1500, np.float32); vp[:, 250:] = 3000
+    rho = np.full((nx, nz), 1000, np.float32); rho[:, 250:] = 2000
+    rec = np.stack([np.arange(0, nx, 7) * 10.0, np.full(len(range(0, nx, 7)), 20.0)], 1)
+    grid = sam.Grid(nx, nz, 10.0, 10.0, 4.0, 0.001, 0.001, 15.0, npml=50)
+    names = sam.variants()
+    shot = sam.Shot(10.0 * 2460, 20.0, rec)
+    print("[gpu] config1 full record, tuned vs single-step", flush=True)
+    with sam.Modeller(grid, vp, rho) as m:
+        best = m.autotune(steps=6)
+        tuned = m.forward([shot])[0]
+        print(f"[gpu] tuned {names[best]} order {m.get_option('tile_order')} "
+              f"chunk {m.get_option('chunk')}", flush=True)
+        m.set_option("variant", names.index("waves4_chunk32_pf2"))
+        single = m.forward([shot])[0]
+    assert tuned.shape == (len(rec), 4000)
+    assert np.all(np.isfinite(tuned))
+    assert np.array_equal(tuned.view(np.int32), single.view(np.int32))
+    # the direct wave (1500 m/s) reaches every receiver within 5 km of the
+    # source inside the 4 s record
+    near = np.abs(rec[:, 0] - shot.sx) < 5000.0
+    assert np.all(np.abs(tuned[near]).max(axis=1) > 0)
+
+
+def test_config4_window_full_length_gradient_invariants():
+    # One 2400-column window of the config 2/4 shape (3201 x 1201, dx 7.62,
+    # sponge 60, 8000 steps), smooth model with a fast inclusion.
+    nx, nz, h = 3201, 1201, 7.62
+    x = np.arange(nx)[:, None] * h; z = np.arange(nz)[None, :] * h
+    vp = (1500 + 0.5 * z + 2500 * (np.hypot(x - 12000, z - 5500) < 1500)).astype(np.float32)
+    rho = (310.0 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+    grid = sam.Grid(nx, nz, h, h, 4.0, 0.0005, 0.0005, 15.0, npml=60)
+    wx0, w = 400, 2400
+    rec = np.stack([(wx0 + np.arange(w)) * h + 0.5 * h, np.full(w, 15.24)], 1).astype(np.float32)
+    shot = sam.Shot((wx0 + w // 2 + 0.5) * h, 15.24, rec)
+    rng = np.random.default_rng(12)
+    print("[gpu] config4 window gradient, k = 200 / 57, one- and two-step adjoint", flush=True)
+    with sam.Modeller(grid, vp, rho) as m:
+        m.autotune(windows=[(wx0, w)], nshots=1, steps=6)
+        d = m.forward_windows([shot], [(wx0, w)])[0]
+        adj = [(d * (1 + 0.1 * rng.standard_normal(d.shape))).astype(np.float32)]
+        g200 = m.gradient([shot], adj, ckpt_k=200, windows=[(wx0, w)])
+        g57 = m.gradient([shot], adj, ckpt_k=57, windows=[(wx0, w)])
+        m.set_option("adjoint_steps", 1)
+        g1 = m.gradient([shot], adj, ckpt_k=200, windows=[(wx0, w)])
+    assert np.all(np.isfinite(g200[0])) and np.all(np.isfinite(g200[1]))
+    assert np.abs(g200[0]).max() > 0 and np.abs(g200[1]).max() > 0
+    # nothing outside the window is touched
+    assert not np.any(g200[0][:wx0]) and not np.any(g200[0][wx0 + w:])
+    for g in (g57, g1):
+        assert np.array_equal(g[0], g200[0]) and np.array_equal(g[1], g200[1])
