@@ -437,6 +437,9 @@ def main():
     ap.add_argument("--max-batch", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="profiling runs only: shorten each shot to this many time steps")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="memory-system ceiling of the kernels' access pattern on the 5100 x "
+                         "5100 grid (four data layouts, chunks 64 and 80), then exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="profiling calibration: one 1 GiB device-to-device copy, then exit")
     ap.add_argument("--tuning", action="store_true", help="print per-variant autotune times")
@@ -448,6 +451,14 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.stream_probe:
+        import subprocess
+        exe = os.path.join(ROOT, "seisacomod2d_amd", "stream_probe")
+        if not os.path.exists(exe):
+            sys.exit("stream_probe is not built (python -m seisacomod2d_amd.build)")
+        for chunk in (64, 80):
+            subprocess.run([exe, str(chunk), "4"], check=True)
         return
     import seisacomod2d_amd as sam
     {"plain": bench_plain, "batched": bench_batched, "gradient": bench_gradient}[args.workload](args, sam)

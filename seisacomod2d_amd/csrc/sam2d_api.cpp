@@ -791,45 +791,71 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
         ctx->tune_ms[v] = ms / steps;
         if (ms < best_ms) { best_ms = ms; best = v; }
     }
-    // Second pass for a multi-step winner: its march length (the variant's
-    // chunk scaled by 1/2 .. 3/2, multiples of 4) jointly with the
-    // chunk-column order (stride 1 = in order, 2..9).  Each pair is timed
-    // twice over four times the steps (they differ by a few per cent) and the
-    // fastest kept.
-    if (variant_steps(best) > 1) {
-        int w0, c0;
-        variant_shape(best, &w0, &c0);
-        std::vector<int> chunks;
-        for (int eighths : {4, 5, 6, 7, 8, 9, 10, 12}) {
-            const int c = std::max(16, (c0 * eighths / 8 + 2) / 4 * 4);
-            if (std::find(chunks.begin(), chunks.end(), c) == chunks.end()) chunks.push_back(c);
+    // Second pass, for the three fastest multi-step variants: the march
+    // length (the variant's chunk scaled by 1/2 .. 3/2, multiples of 4)
+    // jointly with the chunk-column order (stride 1 = in order, 2..9), each
+    // combination timed twice over four times the steps.  The four fastest
+    // combinations are then timed three times over twelve times the steps
+    // (they differ by a few per cent, as much as a short timing varies) and
+    // the fastest is kept.
+    struct Combo {
+        double ms;
+        int variant, chunk, order;
+    };
+    auto time_combo = [&](const Combo &cb, int nsteps, int reps, double *out) -> int {
+        ctx->chunk = cb.chunk;
+        ctx->tile_order = cb.order;
+        int ntiles = 0;
+        SAM2D_TRY(stage_batch(ctx, plans, cb.variant, &ntiles));
+        double t = 1e300;
+        for (int rep = 0; rep < reps; rep++) {
+            SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, cb.variant, 0, 2, 0, 0, nullptr));   // warm
+            HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+            SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, cb.variant, 2, 2 + nsteps, 0, 0, nullptr));
+            HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+            HIP_TRY(hipEventSynchronize(ctx->ev1));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            t = std::min(t, (double)ms);
         }
-        const int osteps = 4 * steps;
-        double best_o = 1e300;
-        int best_order = 0, best_chunk = 0;
-        for (int c : chunks) {
-            ctx->chunk = c == c0 ? 0 : c;
-            int ntiles = 0;
-            SAM2D_TRY(stage_batch(ctx, plans, best, &ntiles));
-            for (int xo = 1; xo <= 9; xo++) {
-                ctx->tile_order = xo;
-                double t = 1e300;
-                for (int rep = 0; rep < 2; rep++) {
-                    SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, best, 0, 2, 0, 0, nullptr));   // warm
-                    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-                    SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, best, 2, 2 + osteps, 0, 0, nullptr));
-                    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-                    HIP_TRY(hipEventSynchronize(ctx->ev1));
-                    float ms = 0;
-                    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-                    t = std::min(t, (double)ms);
-                }
-                if (t < best_o) { best_o = t; best_order = xo; best_chunk = ctx->chunk; }
+        *out = t / nsteps;
+        return SAM2D_OK;
+    };
+    std::vector<int> multi;
+    for (int v = 0; v < num_variants(); v++)
+        if (variant_steps(v) > 1) multi.push_back(v);
+    std::sort(multi.begin(), multi.end(),
+              [&](int a, int b) { return ctx->tune_ms[a] < ctx->tune_ms[b]; });
+    if (variant_steps(best) > 1 && !multi.empty()) {
+        if (multi.size() > 3) multi.resize(3);
+        std::vector<Combo> combos;
+        for (int v : multi) {
+            int w0, c0;
+            variant_shape(v, &w0, &c0);
+            std::vector<int> chunks;
+            for (int eighths : {4, 5, 6, 7, 8, 9, 10, 12}) {
+                const int c = std::max(16, (c0 * eighths / 8 + 2) / 4 * 4);
+                if (std::find(chunks.begin(), chunks.end(), c) == chunks.end())
+                    chunks.push_back(c);
             }
+            for (int c : chunks)
+                for (int xo = 1; xo <= 9; xo++) {
+                    Combo cb{0.0, v, c == c0 ? 0 : c, xo};
+                    SAM2D_TRY(time_combo(cb, 4 * steps, 2, &cb.ms));
+                    combos.push_back(cb);
+                }
         }
-        ctx->tile_order = best_order;
-        ctx->chunk = best_chunk;
-        ctx->tune_ms[best] = best_o / osteps;
+        std::sort(combos.begin(), combos.end(),
+                  [](const Combo &a, const Combo &b) { return a.ms < b.ms; });
+        if (combos.size() > 4) combos.resize(4);
+        for (Combo &cb : combos) SAM2D_TRY(time_combo(cb, 12 * steps, 3, &cb.ms));
+        const Combo win = *std::min_element(
+            combos.begin(), combos.end(),
+            [](const Combo &a, const Combo &b) { return a.ms < b.ms; });
+        best = win.variant;
+        ctx->chunk = win.chunk;
+        ctx->tile_order = win.order;
+        ctx->tune_ms[best] = win.ms;
     }
     ctx->tuned_variant = best;
     if (best_variant) *best_variant = best;
