@@ -340,7 +340,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         auto psum = [&](int slot) -> int { return slot <= ring_end && slot % 3 != 0 ? 1 : 0; };
         for (; n < n1; n++)
             HIP_TRY(launch_step(variant, ctx->d_desc, ns, ntiles, ctx->grid_cap, n, n - n0,
-                                n - n0 + 1, -1, ctx->stream));
+                                n - n0 + 1, -1, false, ctx->stream));
         int n_top = n1 - 1;
         if (two_step)
             for (; n_top - 1 >= n0; n_top -= 2) {
@@ -440,6 +440,13 @@ extern "C" int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshot
         batch = std::max(1, fit);
     }
     std::vector<double> gv((size_t)ctx->grid.nx * ctx->grid.nz, 0.0), gr(gv.size(), 0.0);
+    // The adjoint reads single fields of the forward states: the forward
+    // sweep and the recomputation keep one array per field.
+    struct LayoutScope {
+        sam2d_ctx *c;
+        explicit LayoutScope(sam2d_ctx *x) : c(x) { c->state_aos = false; }
+        ~LayoutScope() { c->state_aos = true; }
+    } layout(ctx);
     for (int s0 = 0; s0 < nshots; s0 += batch) {
         const int s1 = std::min(nshots, s0 + batch);
         std::vector<ShotPlan> sub(plans.begin() + s0, plans.begin() + s1);

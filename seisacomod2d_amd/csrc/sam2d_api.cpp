@@ -238,7 +238,8 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
         DeviceDomain *dom = p.dom;
         const int nnz = dom->host.nnz, nnx = dom->host.nnx;
         const size_t N = (size_t)nnz * nnx;
-        if (variant_steps(variant) > 1 && N >= ((size_t)1 << 30))
+        // 32-bit byte offsets: 16 N bytes of cell-interleaved state, 4 N per field
+        if (variant_steps(variant) > 1 && N >= ((size_t)1 << (ctx->state_aos ? 28 : 30)))
             return fail(SAM2D_ERR_PARAM, "grid of %zu cells too large for a multi-step variant "
                         "(32-bit byte offsets); choose a single-step variant", N);
         if (!ctx->bufs[s]) ctx->bufs[s].reset(new ShotBuffers());
@@ -350,7 +351,8 @@ int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int ngroups, int nshot
                 if (rec[j] >= 0) mask |= 1 << j;
             }
             HIP_TRY(launch_stepn(variant, desc, nshots, ntiles, ctx->grid_cap, kza, n, cur, cur ^ 1,
-                                 mask, band_w, pend, pend_state, max_ng, order, ctx->stream));
+                                 mask, band_w, pend, pend_state, max_ng, order, ctx->state_aos,
+                                 ctx->stream));
             for (int j = 0; j < BAND_SLICES; j++) pend[j] = rec[j];
             pend_state = rec_after(ta, n + T - 1);
             cur ^= 1;
@@ -360,27 +362,28 @@ int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int ngroups, int nshot
             // Fewer than T steps left: flush, then single steps (each kernel
             // gathers the step before it).
             HIP_TRY(launch_gather(desc, nshots, max_ng, band_w ^ 1, pend, cur, pend_state,
-                                  ctx->stream));
+                                  ctx->state_aos, ctx->stream));
             for (int j = 0; j < BAND_SLICES; j++) pend[j] = -1;
             for (bool first = true; n < n1; n++, first = false) {
                 HIP_TRY(launch_step(single, desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
-                                    first ? -1 : rec_slot_for_kernel(ta, n), ctx->stream));
+                                    first ? -1 : rec_slot_for_kernel(ta, n), ctx->state_aos,
+                                    ctx->stream));
                 cur ^= 1;
             }
             pend_state = rec_after(ta, n1 - 1);
         }
         HIP_TRY(launch_gather(desc, nshots, max_ng, band_w ^ 1, pend, cur, pend_state,
-                              ctx->stream));
+                              ctx->state_aos, ctx->stream));
     } else {
         for (; n < n1; n++) {
             HIP_TRY(launch_step(variant, desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
-                                rec_slot_for_kernel(ta, n), ctx->stream));
+                                rec_slot_for_kernel(ta, n), ctx->state_aos, ctx->stream));
             cur ^= 1;
         }
         if (n1 == ta.fd_nt) {
             const int slot = rec_slot_for_kernel(ta, n1);
             if (slot >= 0 && max_ng > 0)
-                HIP_TRY(launch_record(desc, nshots, max_ng, cur, slot, ctx->stream));
+                HIP_TRY(launch_record(desc, nshots, max_ng, cur, slot, ctx->state_aos, ctx->stream));
         }
     }
     if (end_slot) *end_slot = cur;
@@ -398,7 +401,7 @@ int run_time_loop(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int max_n
     bool used_graph = false;
     if (ctx->use_graph) {
         GraphKey key{nshots, ntiles, variant, max_ng, ctx->grid_cap, ctx->tile_order,
-                     tile_chunk(ctx, variant)};
+                     tile_chunk(ctx, variant), ctx->state_aos};
         hipGraphExec_t exec = nullptr;
         auto it = ctx->graphs.find(key);
         if (it != ctx->graphs.end()) exec = it->second;
@@ -492,7 +495,10 @@ static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, flo
                                    ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             const int nnz = one[0].dom->host.nnz;
-            for (int f = 0; f < 4; f++) std::fill(&keep[f * N], &keep[f * N] + nnz, 0.0f);
+            if (ctx->state_aos)      // cells 0 .. nnz-1, four fields each
+                std::fill(keep.begin(), keep.begin() + 4 * (size_t)nnz, 0.0f);
+            else
+                for (int f = 0; f < 4; f++) std::fill(&keep[f * N], &keep[f * N] + nnz, 0.0f);
         }
         int ntiles = 0;
         SAM2D_TRY(stage_batch(ctx, one, variant, &ntiles));

@@ -75,11 +75,12 @@ struct ShotPlan {
 
 struct GraphKey {
     int nshots, ntiles, variant, max_ng, grid_cap, tile_order, chunk;
+    bool aos;
     bool operator<(const GraphKey &o) const
     {
-        return std::tie(nshots, ntiles, variant, max_ng, grid_cap, tile_order, chunk) <
+        return std::tie(nshots, ntiles, variant, max_ng, grid_cap, tile_order, chunk, aos) <
                std::tie(o.nshots, o.ntiles, o.variant, o.max_ng, o.grid_cap, o.tile_order,
-                        o.chunk);
+                        o.chunk, o.aos);
     }
 };
 
@@ -114,6 +115,8 @@ struct sam2d_ctx {
     int tile_order = 0;                    // chunk-column stride of multi-step kernels (tuned)
     int adjoint_tile_order = 7;            // chunk-column stride of the adjoint kernels
     int chunk = 0;                         // march length of multi-step kernels, 0 = variant's
+    bool state_aos = true;                 // cell-interleaved state (forward calls); the
+                                           // gradient runs with one array per field
     std::vector<double> tune_ms;           // per variant, from the autotuner
     sam2d_stats stats = sam2d_stats();
     sam2d::AdjointState *adj = nullptr;

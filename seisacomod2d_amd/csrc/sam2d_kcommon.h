@@ -31,6 +31,11 @@ __device__ __forceinline__ void dplus_coef(int i, int n, float &c0, float &c1)
 // Flat accesses also count against lgkmcnt, so every lane-shuffle wait would
 // otherwise drain all loads in flight and serialise the column prefetch.
 #define SAM2D_GLOBAL __attribute__((address_space(1)))
+
+// The four state fields of a cell (px, pz, vx, vz) in the cell-interleaved
+// state layout: a native vector type, which global-address-space pointers
+// accept.
+typedef float fquad __attribute__((ext_vector_type(4)));
 template <class T>
 __device__ __forceinline__ SAM2D_GLOBAL T *gp(T *p) { return (SAM2D_GLOBAL T *)p; }
 
@@ -108,14 +113,16 @@ __device__ __forceinline__ SAM2D_GLOBAL float *band_slice(const ShotDev &d, int 
 // after cpu_transpose, :370-381): samples bs.x .. bs.w from band slices 0..3
 // of parity band_r, sample ss as px + pz of the state slot (each skipped when
 // < 0).
+template <bool AOS>
 __device__ __forceinline__ void gather_deferred(const ShotDev *sh, int nshots, int band_r, int4 bs,
                                                 int state_slot, int ss, int gtid, int gstride)
 {
     const int smp[BAND_SLICES] = {bs.x, bs.y, bs.z, bs.w};
     for (int s = 0; s < nshots; ++s) {
         const ShotDev &d = sh[s];
-        const SAM2D_GLOBAL float *px = gp(slot_field(d, state_slot, 0));
-        const SAM2D_GLOBAL float *pz = gp(slot_field(d, state_slot, 1));
+        const SAM2D_GLOBAL float *px = gp(AOS ? slot_cells(d, state_slot) : slot_field(d, state_slot, 0));
+        const SAM2D_GLOBAL float *pz = gp(AOS ? slot_cells(d, state_slot) + 1 : slot_field(d, state_slot, 1));
+        const int fs = AOS ? 4 : 1;                   // floats between cells
         const SAM2D_GLOBAL int *gc = gp(d.gcell);
         SAM2D_GLOBAL float *seis = gp(d.seis);
         for (int g = gtid; g < d.ng; g += gstride) {
@@ -126,7 +133,7 @@ __device__ __forceinline__ void gather_deferred(const ShotDev *sh, int nshots, i
             #pragma unroll
             for (int j = 0; j < BAND_SLICES; j++)
                 if (smp[j] >= 0) tr[smp[j]] = band_slice(d, band_r, j)[at];
-            if (ss >= 0) tr[ss] = px[c] + pz[c];
+            if (ss >= 0) tr[ss] = px[(size_t)fs * c] + pz[(size_t)fs * c];
         }
     }
 }

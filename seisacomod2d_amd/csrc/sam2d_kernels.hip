@@ -31,17 +31,19 @@ namespace sam2d {
 // Receiver gather of p = px + pz from the given state buffers
 // (cpu_record, cpu_modelling_kernels.cpp:24-31, trace-major output
 // as after cpu_transpose, :370-381).
+template <bool AOS>
 __device__ __forceinline__ void gather_receivers(const ShotDev *sh, int nshots, int in_slot,
                                                  int slot, int gtid, int gstride)
 {
     for (int s = 0; s < nshots; ++s) {
         const ShotDev &d = sh[s];
-        const SAM2D_GLOBAL float *px = gp(slot_field(d, in_slot, 0));
-        const SAM2D_GLOBAL float *pz = gp(slot_field(d, in_slot, 1));
+        const SAM2D_GLOBAL float *px = gp(AOS ? slot_cells(d, in_slot) : slot_field(d, in_slot, 0));
+        const SAM2D_GLOBAL float *pz = gp(AOS ? slot_cells(d, in_slot) + 1 : slot_field(d, in_slot, 1));
+        const size_t fs = AOS ? 4 : 1;               // floats between cells
         const SAM2D_GLOBAL int *gc = gp(d.gcell);
         SAM2D_GLOBAL float *seis = gp(d.seis);
         for (int g = gtid; g < d.ng; g += gstride) {
-            int c = gc[g];
+            const size_t c = fs * (size_t)gc[g];
             seis[(size_t)g * d.real_nt + slot] = px[c] + pz[c];
         }
     }
@@ -54,11 +56,23 @@ struct ColOps {
     float dax, skx;             // x-profiles at column c (uniform)
     float bx, bz, vz_j;         // BX, BZ and vz at column j = c-2
     float dbx, sbx;             // x-profiles at column j (uniform)
+    fquad s;                    // cell-interleaved state only: state at column c+1
 };
 
 template <bool NT>
 __device__ __forceinline__ void st(SAM2D_GLOBAL float *p, unsigned i, float v)
 {
+    if (NT) __builtin_nontemporal_store(v, p + i);
+    else p[i] = v;
+}
+
+// Store of a cell's four state fields (cell-interleaved layout).
+template <bool NT>
+__device__ __forceinline__ void st4(SAM2D_GLOBAL fquad *p, unsigned i, float a, float b, float c,
+                                    float e)
+{
+    fquad v;
+    v.x = a; v.y = b; v.z = c; v.w = e;
     if (NT) __builtin_nontemporal_store(v, p + i);
     else p[i] = v;
 }
@@ -79,7 +93,12 @@ __device__ __forceinline__ void st(SAM2D_GLOBAL float *p, unsigned i, float v)
 // (halo rows above row 0 or below nnz-1, columns outside [0, nnx)) reads a
 // finite in-grid value, and every such value meets a zero stencil coefficient
 // (the reference's edge rules) before it could reach an owned cell.
-template <int WAVES, int PF, bool NTS>
+//
+// AOS: cell-interleaved state slots (slot_cells): one 16-byte load per column
+// brings the state of column c+1, whose px, pz, vz wait in windows; the
+// v-phase at j stores all four fields of column j, px and pz kept from its
+// p-phase two columns earlier.
+template <int WAVES, int PF, bool NTS, bool AOS>
 __global__ __launch_bounds__(WAVES * 64)
 void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
             int ntiles, int step, int in_slot, int out_slot, int rec_slot)
@@ -89,7 +108,7 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
     const int wv = threadIdx.x >> 6;
 
     if (rec_slot >= 0)
-        gather_receivers(shots, nshots, in_slot, rec_slot,
+        gather_receivers<AOS>(shots, nshots, in_slot, rec_slot,
                          blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 
     for (int v = blockIdx.x; v < vtotal; v += gridDim.x) {
@@ -118,6 +137,8 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         SAM2D_GLOBAL float *__restrict__ pzo = gp(slot_field(d, out_slot, 1));
         SAM2D_GLOBAL float *__restrict__ vxo = gp(slot_field(d, out_slot, 2));
         SAM2D_GLOBAL float *__restrict__ vzo = gp(slot_field(d, out_slot, 3));
+        const SAM2D_GLOBAL fquad *__restrict__ si = (const SAM2D_GLOBAL fquad *)gp(slot_cells(d, in_slot));
+        SAM2D_GLOBAL fquad *__restrict__ so = (SAM2D_GLOBAL fquad *)gp(slot_cells(d, out_slot));
         const SAM2D_GLOBAL float *__restrict__ KX = gp(d.KX);
         const SAM2D_GLOBAL float *__restrict__ KZ = gp(d.KZ);
         const SAM2D_GLOBAL float *__restrict__ BX = gp(d.BX);
@@ -142,10 +163,15 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         auto load_col = [&](int c) -> ColOps {
             ColOps o;
             const unsigned b1 = colbase(c + 1), b0 = colbase(c), bj = colbase(c - 2);
-            o.vx_next = vxi[b1 + rc];
-            o.vz = vzi[b0 + rc]; o.px = pxi[b0 + rc]; o.pz = pzi[b0 + rc];
+            if constexpr (AOS) {
+                o.s = si[b1 + rc];
+            } else {
+                o.vx_next = vxi[b1 + rc];
+                o.vz = vzi[b0 + rc]; o.px = pxi[b0 + rc]; o.pz = pzi[b0 + rc];
+                o.vz_j = vzi[bj + rc];
+            }
             o.kx = KX[b0 + rc]; o.kz = KZ[b0 + rc];
-            o.bx = BX[bj + rc]; o.bz = BZ[bj + rc]; o.vz_j = vzi[bj + rc];
+            o.bx = BX[bj + rc]; o.bz = BZ[bj + rc];
             const int cc = min(max(c, 0), nnx - 1), jc = min(max(c - 2, 0), nnx - 1);
             o.dax = prx[cc]; o.skx = prx[nnx + cc];
             o.dbx = prx[2 * nnx + jc]; o.sbx = prx[3 * nnx + jc];
@@ -154,12 +180,21 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
 
         // Windows, indexed modulo 4 relative to the unrolled block's base
         // column c: vxw[k & 3] = vx at c-2+k; pw[k & 3] = new p at c-3+k.
-        float vxw[4], pw[4];
-        vxw[0] = vxi[colbase(xa - 3) + rc];
-        vxw[1] = vxi[colbase(xa - 2) + rc];
-        vxw[2] = vxi[colbase(xa - 1) + rc];
-        vxw[3] = 0.0f;
-        pw[0] = pw[1] = pw[2] = pw[3] = 0.0f;
+        // AOS also: pxw, pzw, vzw (old px, pz, vz, like vxw) and pxnw, pznw
+        // (new px, pz, like pw).
+        float vxw[4], pw[4], pxw[4], pzw[4], vzw[4], pxnw[4], pznw[4];
+        #pragma unroll
+        for (int k = 0; k < 4; k++) vxw[k] = pw[k] = pxw[k] = pzw[k] = vzw[k] = pxnw[k] = pznw[k] = 0.0f;
+        #pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const unsigned b = colbase(xa - 3 + k) + rc;
+            if constexpr (AOS) {
+                const fquad sv = si[b];
+                pxw[k] = sv.x; pzw[k] = sv.y; vxw[k] = sv.z; vzw[k] = sv.w;
+            } else {
+                vxw[k] = vxi[b];
+            }
+        }
 
         ColOps q[PF];
         #pragma unroll
@@ -173,22 +208,34 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
                 const int c = c0 + u;
                 ColOps &cur = q[u % PF];
                 // ---- p-phase at column c (cpu_pml_p, :335-351; source, :135-140)
-                vxw[(u + 3) & 3] = cur.vx_next;
+                if constexpr (AOS) {
+                    pxw[(u + 3) & 3] = cur.s.x;
+                    pzw[(u + 3) & 3] = cur.s.y;
+                    vxw[(u + 3) & 3] = cur.s.z;
+                    vzw[(u + 3) & 3] = cur.s.w;
+                } else {
+                    vxw[(u + 3) & 3] = cur.vx_next;
+                }
                 const float vx_m2 = vxw[u & 3], vx_m1 = vxw[(u + 1) & 3];
                 const float vx_0 = vxw[(u + 2) & 3], vx_p1 = vxw[(u + 3) & 3];
                 float xm0, xm1;
                 dminus_coef(c, nnx, xm0, xm1);
                 const float dvx = fmaf(xm0, vx_0 - vx_m1, xm1 * (vx_p1 - vx_m2));
-                const float vz_0 = cur.vz;
+                const float vz_0 = AOS ? vzw[(u + 2) & 3] : cur.vz;
                 const float vz_u2 = shfl_up(vz_0, 2), vz_u1 = shfl_up(vz_0, 1);
                 const float vz_d1 = shfl_dn(vz_0, 1);
                 const float dvz = fmaf(zm0, vz_0 - vz_u1, zm1 * (vz_d1 - vz_u2));
-                float pxn = fmaf(cur.kx * cur.skx, dvx, cur.dax * cur.px);
-                const float pzn = fmaf(cur.kz * skz, dvz, daz * cur.pz);
+                const float px_old = AOS ? pxw[(u + 2) & 3] : cur.px;
+                const float pz_old = AOS ? pzw[(u + 2) & 3] : cur.pz;
+                float pxn = fmaf(cur.kx * cur.skx, dvx, cur.dax * px_old);
+                const float pzn = fmaf(cur.kz * skz, dvz, daz * pz_old);
                 if (src_row && c == src_x) pxn += amp;
                 const float pn = pxn + pzn;
                 pw[(u + 3) & 3] = pn;
-                if (rown && c >= xa && c < xb) {
+                if (AOS) {
+                    pxnw[(u + 3) & 3] = pxn;          // stored with vx, vz at j
+                    pznw[(u + 3) & 3] = pzn;
+                } else if (rown && c >= xa && c < xb) {
                     const unsigned id = (unsigned)c * (unsigned)nnz + (unsigned)r;
                     st<NTS>(pxo, id, pxn);
                     st<NTS>(pzo, id, pzn);
@@ -205,11 +252,16 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
                     const float q_d1 = shfl_dn(p_m2, 1), q_d2 = shfl_dn(p_m2, 2);
                     const float dpz = fmaf(zp0, q_d1 - p_m2, zp1 * (q_d2 - q_u1));
                     const float vxn = fmaf(cur.bx * cur.sbx, dpx, cur.dbx * vx_m2);
-                    const float vzn = fmaf(cur.bz * sbz, dpz, dbz * cur.vz_j);
+                    const float vz_j = AOS ? vzw[u & 3] : cur.vz_j;
+                    const float vzn = fmaf(cur.bz * sbz, dpz, dbz * vz_j);
                     if (rown) {
                         const unsigned id = (unsigned)j * (unsigned)nnz + (unsigned)r;
-                        st<NTS>(vxo, id, vxn);
-                        st<NTS>(vzo, id, vzn);
+                        if constexpr (AOS) {
+                            st4<NTS>(so, id, pxnw[(u + 1) & 3], pznw[(u + 1) & 3], vxn, vzn);
+                        } else {
+                            st<NTS>(vxo, id, vxn);
+                            st<NTS>(vzo, id, vzn);
+                        }
                     }
                 }
                 cur = load_col(c + PF);
@@ -219,10 +271,11 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
 }
 
 // Receiver gather alone (the last step's samples).
+template <bool AOS>
 __global__ void k_record(const ShotDev *__restrict__ shots, int nshots, int in_slot, int slot)
 {
-    gather_receivers(shots, nshots, in_slot, slot, blockIdx.x * blockDim.x + threadIdx.x,
-                     gridDim.x * blockDim.x);
+    gather_receivers<AOS>(shots, nshots, in_slot, slot, blockIdx.x * blockDim.x + threadIdx.x,
+                          gridDim.x * blockDim.x);
 }
 
 // Fill with a constant.  Used instead of hipMemsetAsync so that clearing is
@@ -329,26 +382,28 @@ void variant_name(int v, char *buf, int n)
     else snprintf(buf, n, "waves%d_chunk%d_pf%d%s", w, c, pf, nts ? "_nts" : "");
 }
 
-template <int W, int PF, bool NTS>
+template <int W, int PF, bool NTS, bool AOS>
 static void launch_t(dim3 g, hipStream_t s, const ShotDev *d, int ns, int vt, int per, int nt,
                      int step, int in, int out, int slot)
 {
-    hipLaunchKernelGGL((k_step<W, PF, NTS>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step, in,
-                       out, slot);
+    hipLaunchKernelGGL((k_step<W, PF, NTS, AOS>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step,
+                       in, out, slot);
 }
 
-template <int W>
+template <int W, bool AOS>
 static void launch_w(int pf, int nts, dim3 g, hipStream_t s, const ShotDev *d, int ns, int vt,
                      int per, int nt, int step, int in, int out, int slot)
 {
-    if (pf == 2 && !nts) launch_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, in, out, slot);
-    else if (pf == 2) launch_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, in, out, slot);
-    else if (!nts) launch_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, in, out, slot);
-    else launch_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, in, out, slot);
+    if (pf == 2 && !nts) launch_t<W, 2, false, AOS>(g, s, d, ns, vt, per, nt, step, in, out, slot);
+    else if (pf == 2) launch_t<W, 2, true, AOS>(g, s, d, ns, vt, per, nt, step, in, out, slot);
+    else if (!nts) launch_t<W, 4, false, AOS>(g, s, d, ns, vt, per, nt, step, in, out, slot);
+    else launch_t<W, 4, true, AOS>(g, s, d, ns, vt, per, nt, step, in, out, slot);
 }
 
-hipError_t launch_step(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
-                       int step, int in_slot, int out_slot, int rec_slot, hipStream_t stream)
+template <bool AOS>
+static hipError_t launch_step_l(int variant, const ShotDev *d_shots, int nshots, int ntiles,
+                                int grid_cap, int step, int in_slot, int out_slot, int rec_slot,
+                                hipStream_t stream)
 {
     int waves, chunk, pf, nts;
     decode(variant, waves, chunk, pf, nts);
@@ -358,21 +413,36 @@ hipError_t launch_step(int variant, const ShotDev *d_shots, int nshots, int ntil
     if (grid_cap > 0 && grid > grid_cap) grid = (grid_cap + 7) / 8 * 8;
     const dim3 g(grid);
     switch (waves) {
-    case 1: launch_w<1>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
-    case 2: launch_w<2>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
-    case 4: launch_w<4>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
-    case 8: launch_w<8>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
+    case 1: launch_w<1, AOS>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
+    case 2: launch_w<2, AOS>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
+    case 4: launch_w<4, AOS>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
+    case 8: launch_w<8, AOS>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+hipError_t launch_step(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
+                       int step, int in_slot, int out_slot, int rec_slot, bool aos,
+                       hipStream_t stream)
+{
+    return aos ? launch_step_l<true>(variant, d_shots, nshots, ntiles, grid_cap, step, in_slot,
+                                     out_slot, rec_slot, stream)
+               : launch_step_l<false>(variant, d_shots, nshots, ntiles, grid_cap, step, in_slot,
+                                      out_slot, rec_slot, stream);
+}
+
 hipError_t launch_record(const ShotDev *d_shots, int nshots, int max_ng, int in_slot, int slot,
-                         hipStream_t stream)
+                         bool aos, hipStream_t stream)
 {
     int blocks = (max_ng + 255) / 256;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_record, dim3(blocks), dim3(256), 0, stream, d_shots, nshots, in_slot, slot);
+    if (aos)
+        hipLaunchKernelGGL(k_record<true>, dim3(blocks), dim3(256), 0, stream, d_shots, nshots,
+                           in_slot, slot);
+    else
+        hipLaunchKernelGGL(k_record<false>, dim3(blocks), dim3(256), 0, stream, d_shots, nshots,
+                           in_slot, slot);
     return hipGetLastError();
 }
 

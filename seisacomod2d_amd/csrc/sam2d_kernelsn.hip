@@ -44,6 +44,8 @@ struct ColN {
     float vx_next;              // vx(n-1) at column c+1
     float vz, px, pz, kx, kz;   // at column c (kz unused when KZ aliases KX)
     float bx, bz, vz_j;         // BX, BZ, vz(n-1) at column c-2
+    fquad s;                    // cell-interleaved state only: the state at column
+                                // c+1, kept whole until the column consumes it
 };
 
 // RING: the states after the intermediate steps are stored as well, into
@@ -52,7 +54,11 @@ struct ColN {
 // state, at the multi-step kernel's read traffic.  The adjoint reads only
 // p = px + pz of a state, so an intermediate slot holds p (the same rounded
 // sum) in place of px, vx and vz, and no pz: 12 B per cell instead of 16.
-template <int T, int WAVES, int PF, bool KZA, bool RING>
+// AOS: the state slots are cell-interleaved (slot_cells): one 16-byte load
+// per column and row brings px, pz, vx, vz of column c+1; px, pz, vz wait a
+// column (vz three) in windows, and the last step stores all four fields of
+// column jv at once, px and pz kept from its p-phase two columns earlier.
+template <int T, int WAVES, int PF, bool KZA, bool RING, bool AOS>
 __global__ __launch_bounds__(WAVES * 64)
 void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_xcd, int ntiles,
              int step, int in_slot, int out_slot, int band_mask, int band_w, int4 band_s,
@@ -60,6 +66,7 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
 {
     static_assert(T >= 2 && T <= BAND_SLICES + 1, "steps per pass");
     static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
+    static_assert(!(AOS && RING), "the recomputation ring is one array per field");
     constexpr int HALO = 3 * T;
     constexpr int OWN = 64 - 6 * T;
     using gcchar = const SAM2D_GLOBAL char;
@@ -68,7 +75,7 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
     if (band_s.x >= 0 || band_s.y >= 0 || band_s.z >= 0 || band_s.w >= 0 || state_sample >= 0)
-        gather_deferred(shots, nshots, band_w ^ 1, band_s, in_slot, state_sample,
+        gather_deferred<AOS>(shots, nshots, band_w ^ 1, band_s, in_slot, state_sample,
                         blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 
     for (int v = blockIdx.x; v < vtotal; v += gridDim.x) {
@@ -99,6 +106,8 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
         gchar *__restrict__ pzo = (gchar *)slot_field(d, out_slot, 1);
         gchar *__restrict__ vxo = (gchar *)slot_field(d, out_slot, 2);
         gchar *__restrict__ vzo = (gchar *)slot_field(d, out_slot, 3);
+        gcchar *__restrict__ si = (gcchar *)slot_cells(d, in_slot);       // AOS
+        gchar *__restrict__ so = (gchar *)slot_cells(d, out_slot);
         gcchar *__restrict__ KX = (gcchar *)d.KX;
         gcchar *__restrict__ KZ = (gcchar *)d.KZ;
         gcchar *__restrict__ BX = (gcchar *)d.BX;
@@ -119,6 +128,15 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
         };
         auto stf = [](gchar *__restrict__ base, unsigned off, float val) {
             *(SAM2D_GLOBAL float *__restrict__)(base + off) = val;
+        };
+        // Cell-interleaved state: byte offset 4x that of one field.
+        auto ld4 = [](gcchar *__restrict__ base, unsigned off) -> fquad {
+            return *(const SAM2D_GLOBAL fquad *__restrict__)(base + off);
+        };
+        auto st4 = [](gchar *__restrict__ base, unsigned off, float a, float b, float c, float e) {
+            fquad v;
+            v.x = a; v.y = b; v.z = c; v.w = e;
+            *(SAM2D_GLOBAL fquad *__restrict__)(base + off) = v;
         };
         // Byte offset of this lane's row in column col (clamped to the grid).
         auto coff = [&](int col) -> unsigned {
@@ -159,11 +177,16 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
         auto load_col = [&](int c) -> ColN {
             ColN o;
             const unsigned o1 = coff(c + 1), o0 = coff(c), oj = coff(c - 2);
-            o.vx_next = ld(vxi, o1);
-            o.vz = ld(vzi, o0); o.px = ld(pxi, o0); o.pz = ld(pzi, o0);
+            if constexpr (AOS) {
+                o.s = ld4(si, 4 * o1);
+            } else {
+                o.vx_next = ld(vxi, o1);
+                o.vz = ld(vzi, o0); o.px = ld(pxi, o0); o.pz = ld(pzi, o0);
+                o.vz_j = ld(vzi, oj);
+            }
             o.kx = ld(KX, o0);
             o.kz = KZA ? 0.0f : ld(KZ, o0);
-            o.bx = ld(BX, oj); o.bz = ld(BZ, oj); o.vz_j = ld(vzi, oj);
+            o.bx = ld(BX, oj); o.bz = ld(BZ, oj);
             return o;
         };
 
@@ -171,12 +194,13 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
         //   vxin          vx(n-1) at c-2 .. c+1
         //   P, PX, PZ[s]  p-phase s results at c-3s-3 .. c-3s (and KXw, KZw)
         //   VX, VZ[s]     v-phase s results at c-3s-5 .. c-3s-2 (and BXw, BZw)
-        float vxin[4];
+        //   pxin, pzin, vzin  (AOS) px, pz, vz(n-1) at c-2 .. c+1
+        float vxin[4], pxin[4], pzin[4], vzin[4];
         float P[T][4], PX[T][4], PZ[T][4], KXw[T][4], KZw[T][4];
         float VX[T][4], VZ[T][4], BXw[T][4], BZw[T][4];
         #pragma unroll
         for (int k = 0; k < 4; k++) {
-            vxin[k] = 0.0f;
+            vxin[k] = pxin[k] = pzin[k] = vzin[k] = 0.0f;
             #pragma unroll
             for (int t = 0; t < T; t++) {
                 P[t][k] = PX[t][k] = PZ[t][k] = KXw[t][k] = KZw[t][k] = 0.0f;
@@ -184,9 +208,17 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
             }
         }
         const int c_first = xa - (3 * T - 2);
-        vxin[0] = ld(vxi, coff(c_first - 2));
-        vxin[1] = ld(vxi, coff(c_first - 1));
-        vxin[2] = ld(vxi, coff(c_first));
+        if constexpr (AOS) {
+            #pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const fquad sv = ld4(si, 4 * coff(c_first - 2 + k));
+                pxin[k] = sv.x; pzin[k] = sv.y; vxin[k] = sv.z; vzin[k] = sv.w;
+            }
+        } else {
+            vxin[0] = ld(vxi, coff(c_first - 2));
+            vxin[1] = ld(vxi, coff(c_first - 1));
+            vxin[2] = ld(vxi, coff(c_first));
+        }
 
         ColN q[PF];
         #pragma unroll
@@ -198,7 +230,14 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
             constexpr int i0 = u & 3, i1 = (u + 1) & 3, i2 = (u + 2) & 3, i3 = (u + 3) & 3;
             const int c = c0 + u;
             ColN &cur = q[u % PF];
-            vxin[i3] = keep(cur.vx_next);
+            if constexpr (AOS) {
+                pxin[i3] = keep(cur.s.x);
+                pzin[i3] = keep(cur.s.y);
+                vxin[i3] = keep(cur.s.z);
+                vzin[i3] = keep(cur.s.w);
+            } else {
+                vxin[i3] = keep(cur.vx_next);
+            }
 
             #pragma unroll
             for (int t = 0; t < T; t++) {
@@ -209,7 +248,7 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                     if (!FAST) dminus_coef(cp, nnx, m0, m1);
                     const float *vxw = t == 0 ? vxin : VX[t > 0 ? t - 1 : 0];
                     const float dvx = fmaf(m0, vxw[i2] - vxw[i1], m1 * (vxw[i3] - vxw[i0]));
-                    const float vz_0 = t == 0 ? cur.vz : VZ[t > 0 ? t - 1 : 0][i2];
+                    const float vz_0 = t == 0 ? (AOS ? vzin[i2] : cur.vz) : VZ[t > 0 ? t - 1 : 0][i2];
                     const float vz_u1 = lane_up1(vz_0), vz_u2 = lane_up1(vz_u1);
                     const float vz_d1 = lane_dn1(vz_0);
                     const float dvz = fmaf(zm0, vz_0 - vz_u1, zm1 * (vz_d1 - vz_u2));
@@ -217,8 +256,8 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                     if (t == 0) {
                         kx = keep(cur.kx);
                         kz = KZA ? kx : keep(cur.kz);
-                        px_old = cur.px;
-                        pz_old = cur.pz;
+                        px_old = AOS ? pxin[i2] : cur.px;
+                        pz_old = AOS ? pzin[i2] : cur.pz;
                     } else {
                         kx = KXw[t - 1][i0];
                         kz = KZA ? kx : KZw[t - 1][i0];
@@ -247,6 +286,9 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                             const unsigned id = (unsigned)cp * (unsigned)nnz * 4u + rc4;
                             stf(mid[t][0], id, P[t][i3]);
                         }
+                    } else if (AOS) {
+                        PX[t][i3] = pxn;                  // stored with vx, vz at jv
+                        PZ[t][i3] = pzn;
                     } else if (rown && cp >= xa && cp < xb) {
                         const unsigned id = (unsigned)cp * (unsigned)nnz * 4u + rc4;
                         stf(pxo, id, pxn);
@@ -268,7 +310,7 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                         bx = keep(cur.bx);
                         bz = keep(cur.bz);
                         vx_old = vxin[i0];
-                        vz_old = cur.vz_j;
+                        vz_old = AOS ? vzin[i0] : cur.vz_j;
                     } else {
                         bx = BXw[t - 1][i0];
                         bz = BZw[t - 1][i0];
@@ -300,8 +342,12 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                         }
                         const float vzn = fmaf(bz * sbz, dpz, dbz * vz_old);
                         const unsigned id = (unsigned)jv * (unsigned)nnz * 4u + rc4;
-                        stf(vxo, id, vxn);
-                        stf(vzo, id, vzn);
+                        if constexpr (AOS) {
+                            st4(so, 4 * id, PX[t][i1], PZ[t][i1], vxn, vzn);
+                        } else {
+                            stf(vxo, id, vxn);
+                            stf(vzo, id, vzn);
+                        }
                     }
                 }
             }
@@ -332,11 +378,12 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
     }
 }
 
+template <bool AOS>
 __global__ void k_gather(const ShotDev *__restrict__ shots, int nshots, int band_r, int4 band_s,
                          int state_slot, int state_sample)
 {
-    gather_deferred(shots, nshots, band_r, band_s, state_slot, state_sample,
-                    blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+    gather_deferred<AOS>(shots, nshots, band_r, band_s, state_slot, state_sample,
+                         blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 struct StepNArgs {
@@ -346,31 +393,31 @@ struct StepNArgs {
     int ss, ng, xo;
 };
 
-template <int T, int W, int PF, bool KZA, bool RING>
+template <int T, int W, int PF, bool KZA, bool RING, bool AOS>
 static void launchn_t(dim3 g, hipStream_t s, const StepNArgs &a)
 {
-    hipLaunchKernelGGL((k_stepn<T, W, PF, KZA, RING>), g, dim3(64 * W), 0, s, a.d, a.ns, a.vt,
+    hipLaunchKernelGGL((k_stepn<T, W, PF, KZA, RING, AOS>), g, dim3(64 * W), 0, s, a.d, a.ns, a.vt,
                        a.per, a.nt, a.step, a.in, a.out, a.bmask, a.bw, a.bs, a.ss, a.ng, a.xo);
 }
 
-template <int T, int W, bool RING>
+template <int T, int W, bool RING, bool AOS>
 static void launchn_w(int pf, bool kza, dim3 g, hipStream_t s, const StepNArgs &a)
 {
-    if (pf == 2 && kza) launchn_t<T, W, 2, true, RING>(g, s, a);
-    else if (pf == 2) launchn_t<T, W, 2, false, RING>(g, s, a);
-    else if (kza) launchn_t<T, W, 4, true, RING>(g, s, a);
-    else launchn_t<T, W, 4, false, RING>(g, s, a);
+    if (pf == 2 && kza) launchn_t<T, W, 2, true, RING, AOS>(g, s, a);
+    else if (pf == 2) launchn_t<T, W, 2, false, RING, AOS>(g, s, a);
+    else if (kza) launchn_t<T, W, 4, true, RING, AOS>(g, s, a);
+    else launchn_t<T, W, 4, false, RING, AOS>(g, s, a);
 }
 
-template <int T, bool RING = false>
+template <int T, bool RING = false, bool AOS = false>
 static hipError_t launchn_waves(int waves, int pf, bool kza, dim3 g, hipStream_t s,
                                 const StepNArgs &a)
 {
     switch (waves) {
-    case 1: launchn_w<T, 1, RING>(pf, kza, g, s, a); break;
-    case 2: launchn_w<T, 2, RING>(pf, kza, g, s, a); break;
-    case 4: launchn_w<T, 4, RING>(pf, kza, g, s, a); break;
-    case 8: launchn_w<T, 8, RING>(pf, kza, g, s, a); break;
+    case 1: launchn_w<T, 1, RING, AOS>(pf, kza, g, s, a); break;
+    case 2: launchn_w<T, 2, RING, AOS>(pf, kza, g, s, a); break;
+    case 4: launchn_w<T, 4, RING, AOS>(pf, kza, g, s, a); break;
+    case 8: launchn_w<T, 8, RING, AOS>(pf, kza, g, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipSuccess;
@@ -379,7 +426,7 @@ static hipError_t launchn_waves(int waves, int pf, bool kza, dim3 g, hipStream_t
 hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
                         bool kz_alias, int step, int in_slot, int out_slot, int band_mask,
                         int band_w, const int *band_samples, int state_sample, int max_ng,
-                        TileOrder order, hipStream_t stream)
+                        TileOrder order, bool aos, hipStream_t stream)
 {
     int waves, chunk, pf;
     variant_shape_pf(variant, &waves, &chunk, &pf);
@@ -399,11 +446,15 @@ hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int nti
     const StepNArgs a{d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot,
                       band_mask, band_w, bs, state_sample, order.ngroups, order.xorder};
     hipError_t e;
-    switch (T) {
-    case 2: e = launchn_waves<2>(waves, pf, kz_alias, g, stream, a); break;
-    case 3: e = launchn_waves<3>(waves, pf, kz_alias, g, stream, a); break;
-    case 4: e = launchn_waves<4>(waves, pf, kz_alias, g, stream, a); break;
-    case 5: e = launchn_waves<5>(waves, pf, kz_alias, g, stream, a); break;
+    switch (T * 2 + (aos ? 1 : 0)) {
+    case 4: e = launchn_waves<2>(waves, pf, kz_alias, g, stream, a); break;
+    case 5: e = launchn_waves<2, false, true>(waves, pf, kz_alias, g, stream, a); break;
+    case 6: e = launchn_waves<3>(waves, pf, kz_alias, g, stream, a); break;
+    case 7: e = launchn_waves<3, false, true>(waves, pf, kz_alias, g, stream, a); break;
+    case 8: e = launchn_waves<4>(waves, pf, kz_alias, g, stream, a); break;
+    case 9: e = launchn_waves<4, false, true>(waves, pf, kz_alias, g, stream, a); break;
+    case 10: e = launchn_waves<5>(waves, pf, kz_alias, g, stream, a); break;
+    case 11: e = launchn_waves<5, false, true>(waves, pf, kz_alias, g, stream, a); break;
     default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
@@ -431,15 +482,19 @@ hipError_t launch_stepn_ring(int variant, const ShotDev *d_shots, int nshots, in
 }
 
 hipError_t launch_gather(const ShotDev *d_shots, int nshots, int max_ng, int band_r,
-                         const int *band_samples, int state_slot, int state_sample,
+                         const int *band_samples, int state_slot, int state_sample, bool aos,
                          hipStream_t stream)
 {
     const int4 bs = make_int4(band_samples[0], band_samples[1], band_samples[2], band_samples[3]);
     if (max_ng <= 0 || (bs.x < 0 && bs.y < 0 && bs.z < 0 && bs.w < 0 && state_sample < 0))
         return hipSuccess;
     const int blocks = (max_ng + 255) / 256;
-    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, stream, d_shots, nshots, band_r, bs,
-                       state_slot, state_sample);
+    if (aos)
+        hipLaunchKernelGGL(k_gather<true>, dim3(blocks), dim3(256), 0, stream, d_shots, nshots,
+                           band_r, bs, state_slot, state_sample);
+    else
+        hipLaunchKernelGGL(k_gather<false>, dim3(blocks), dim3(256), 0, stream, d_shots, nshots,
+                           band_r, bs, state_slot, state_sample);
     return hipGetLastError();
 }
 
