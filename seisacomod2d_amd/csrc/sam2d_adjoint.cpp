@@ -265,13 +265,22 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     for (int c : ncells) max_cells = std::max(max_cells, c);
     const bool kza = ta.dtdx == ta.dtdz;
 
+    // The forward sweep runs on cell-interleaved state where the multi-step
+    // kernels' 32-bit offsets allow it (16 bytes per cell).  Checkpoints are
+    // copies of its state; a restore rearranges one into ring slot 0 as one
+    // array per field.
+    bool sweep_aos = true;
+    for (int s = 0; s < ns; s++) sweep_aos = sweep_aos && ctx->bufs[s]->cells < ((size_t)1 << 28);
     auto save_or_restore = [&](int m, int slot, bool save) -> int {
         for (int s = 0; s < ns; s++) {
             ShotBuffers &fb = *ctx->bufs[s];
-            const size_t bytes = 4 * fb.cells * sizeof(float);
             float *ck = A.bufs[s]->ckpt + (size_t)m * 4 * fb.cells;
-            HIP_TRY(hipMemcpyAsync(save ? ck : fb.slot(slot), save ? fb.slot(slot) : ck, bytes,
-                                   hipMemcpyDeviceToDevice, ctx->stream));
+            if (save || !sweep_aos)
+                HIP_TRY(hipMemcpyAsync(save ? ck : fb.slot(slot), save ? fb.slot(slot) : ck,
+                                       4 * fb.cells * sizeof(float), hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+            else
+                HIP_TRY(launch_cells_to_fields(ck, fb.slot(slot), fb.cells, ctx->stream));
         }
         return SAM2D_OK;
     };
@@ -312,14 +321,17 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
     // 1. forward with checkpoints: checkpoint m holds the state entering
     //    step m*k
+    //    (cell-interleaved state, as the forward-modelling calls)
     const bool want_seis = seis_out != nullptr;
     int cur = 0;
+    ctx->state_aos = sweep_aos;
     for (int m = 0; m < nseg; m++) {
         SAM2D_TRY(save_or_restore(m, cur, true));
         const int n0 = m * k, n1 = std::min(ta.fd_nt, n0 + k);
         SAM2D_TRY(enqueue_steps_on(ctx, fwd_desc, fwd_groups, ns, fwd_tiles, fwd_variant, n0, n1,
                                    want_seis ? max_ng : 0, cur, &cur));
     }
+    ctx->state_aos = false;
     // 2. backward by segments
     int lam = 0;
     for (int m = nseg - 1; m >= 0; m--) {
@@ -440,8 +452,9 @@ extern "C" int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshot
         batch = std::max(1, fit);
     }
     std::vector<double> gv((size_t)ctx->grid.nx * ctx->grid.nz, 0.0), gr(gv.size(), 0.0);
-    // The adjoint reads single fields of the forward states: the forward
-    // sweep and the recomputation keep one array per field.
+    // The adjoint reads single fields of the forward states: the
+    // recomputation keeps one array per field (the forward sweep switches to
+    // the cell-interleaved layout for itself).
     struct LayoutScope {
         sam2d_ctx *c;
         explicit LayoutScope(sam2d_ctx *x) : c(x) { c->state_aos = false; }

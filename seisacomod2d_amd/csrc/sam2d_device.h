@@ -180,5 +180,8 @@ hipError_t launch_step(int variant, const ShotDev *d_shots, int nshots, int ntil
 hipError_t launch_record(const ShotDev *d_shots, int nshots, int max_ng, int slot, int sample,
                          bool aos, hipStream_t stream);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t stream);
+// n cells of cell-interleaved state at src rearranged into one array per
+// field at dst (4 n floats each; src and dst do not overlap).
+hipError_t launch_cells_to_fields(const float *src, float *dst, size_t n, hipStream_t stream);
 
 }  // namespace sam2d

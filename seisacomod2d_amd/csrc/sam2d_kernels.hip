@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <algorithm>
 #include "sam2d_device.h"
 #include "sam2d_kcommon.h"
 
@@ -276,6 +277,29 @@ __global__ void k_record(const ShotDev *__restrict__ shots, int nshots, int in_s
 {
     gather_receivers<AOS>(shots, nshots, in_slot, slot, blockIdx.x * blockDim.x + threadIdx.x,
                           gridDim.x * blockDim.x);
+}
+
+// Cell-interleaved state (px, pz, vx, vz of cell i at 4i..4i+3) to one array
+// per field (field f of cell i at f*n + i).
+__global__ void k_cells_to_fields(const float4 *__restrict__ src, float *__restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const float4 v = src[i];
+        dst[i] = v.x;
+        dst[n + i] = v.y;
+        dst[2 * n + i] = v.z;
+        dst[3 * n + i] = v.w;
+    }
+}
+
+hipError_t launch_cells_to_fields(const float *src, float *dst, size_t n, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_cells_to_fields, dim3((unsigned)blocks), dim3(256), 0, stream,
+                       (const float4 *)src, dst, n);
+    return hipGetLastError();
 }
 
 // Fill with a constant.  Used instead of hipMemsetAsync so that clearing is
