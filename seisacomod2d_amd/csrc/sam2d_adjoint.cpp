@@ -145,6 +145,10 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         ShotPlan &p = plans[s];
         const DomainHost &h = p.dom->host;
         const size_t N = (size_t)h.nnx * h.nnz;
+        // the adjoint kernels address the cell-interleaved adjoint state with
+        // 32-bit byte offsets (16 bytes per cell)
+        if (N >= ((size_t)1 << 28))
+            return fail(SAM2D_ERR_PARAM, "grid of %zu cells too large for the gradient", N);
         if (!A.bufs[s]) A.bufs[s].reset(new AdjShotBuffers());
         AdjShotBuffers &b = *A.bufs[s];
         const size_t ng1 = (size_t)std::max(1, p.ng);

@@ -61,15 +61,15 @@ __global__ void k_inject(const AdjDev *__restrict__ shots, int nshots, int lam, 
     for (int s = 0; s < nshots; ++s) {
         const AdjDev &d = shots[s];
         if (d.rband) continue;                        // injected inside k_adjoint
-        SAM2D_GLOBAL float *Lpx = gp(d.lam + (4 * (size_t)lam + 0) * d.ncell);
-        SAM2D_GLOBAL float *Lpz = gp(d.lam + (4 * (size_t)lam + 1) * d.ncell);
+        // cell-interleaved adjoint state: L_px, L_pz of cell c at 4c, 4c+1
+        SAM2D_GLOBAL float *L = gp(d.lam + 4 * (size_t)lam * d.ncell);
         for (int u = gtid; u < d.ncells; u += gstride) {
             float sum = 0.0f;
             for (int e = d.rptr[u]; e < d.rptr[u + 1]; e++)
                 sum += gp(d.res)[(size_t)d.rid[e] * d.real_nt + rec_k];
-            const int c = d.rcell[u];
-            Lpx[c] += sum;
-            Lpz[c] += sum;
+            const size_t c = 4 * (size_t)d.rcell[u];
+            L[c] += sum;
+            L[c + 1] += sum;
         }
     }
 }
@@ -104,14 +104,18 @@ hipError_t launch_band(const int *bmap, const int *rptr, const int *rid, const f
     return hipGetLastError();
 }
 
+// The adjoint state is cell-interleaved (L_px, L_pz, L_vx, L_vz of cell i at
+// 4i .. 4i+3): a column's state arrives as one 16-byte load per lane at
+// column c+1 (kept whole in the prefetch slot) and leaves as one 16-byte
+// store at the phase-B column, L_px and L_pz kept from phase A.
 struct AdjCol {
-    float lvx_n, bx_n, fvx_n;   // column c+1: L_vx, BX, fwd vx[n-1]
-    float lvz, bz;              // column c:   L_vz, BZ
-    float lpx, lpz;             //             L_px, L_pz
+    fquad lam;                  // column c+1: adjoint state
+    float bx_n, fvx_n;          //             BX, fwd vx[n-1]
+    float bz;                   // column c:   BZ
     float fvz, fpx, fpz;        //             fwd vz[n-1], px[n], pz[n]
     float kx, kz, gk;           //             KX, KZ (unused when aliased), dJ/dkappa
     float res;                  //             residual of this step (band rows)
-    float lvz_j, gbx_j, gbz_j;  // column j:   L_vz, dJ/dbu2, dJ/dbu1
+    float gbx_j, gbz_j;         // column j:   dJ/dbu2, dJ/dbu1
 };
 
 template <int WAVES, int PF, bool KZA>
@@ -148,14 +152,8 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
 
         // Adjoint state, forward ring slots, coefficients and accumulators
         // never overlap.
-        gcchar *__restrict__ Lpx = (gcchar *)(d.lam + (4 * (size_t)lam_in + 0) * N);
-        gcchar *__restrict__ Lpz = (gcchar *)(d.lam + (4 * (size_t)lam_in + 1) * N);
-        gcchar *__restrict__ Lvx = (gcchar *)(d.lam + (4 * (size_t)lam_in + 2) * N);
-        gcchar *__restrict__ Lvz = (gcchar *)(d.lam + (4 * (size_t)lam_in + 3) * N);
-        gchar *__restrict__ Opx = (gchar *)(d.lam + (4 * (size_t)lam_out + 0) * N);
-        gchar *__restrict__ Opz = (gchar *)(d.lam + (4 * (size_t)lam_out + 1) * N);
-        gchar *__restrict__ Ovx = (gchar *)(d.lam + (4 * (size_t)lam_out + 2) * N);
-        gchar *__restrict__ Ovz = (gchar *)(d.lam + (4 * (size_t)lam_out + 3) * N);
+        gcchar *__restrict__ Lin = (gcchar *)(d.lam + 4 * (size_t)lam_in * N);
+        gchar *__restrict__ Lout = (gchar *)(d.lam + 4 * (size_t)lam_out * N);
         // psum & 1: slot fwd_cur holds p in its px field and no pz; the pz
         // loads then read the px field again (a cache hit) and are not used.
         const bool ps_cur = psum & 1;
@@ -177,6 +175,15 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
 
         auto ld = [](gcchar *__restrict__ base, unsigned off) -> float {
             return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
+        };
+        // Adjoint state: byte offset 4x that of one field.
+        auto ld4 = [](gcchar *__restrict__ base, unsigned off) -> fquad {
+            return *(const SAM2D_GLOBAL fquad *__restrict__)(base + off);
+        };
+        auto st4 = [](gchar *__restrict__ base, unsigned off, float a, float b, float c, float e) {
+            fquad v;
+            v.x = a; v.y = b; v.z = c; v.w = e;
+            *(SAM2D_GLOBAL fquad *__restrict__)(base + off) = v;
         };
         auto ldw = [](gchar *__restrict__ base, unsigned off) -> float {
             return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
@@ -215,14 +222,13 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         auto load_col = [&](int c) -> AdjCol {
             AdjCol o;
             const unsigned o1 = coff(c + 1), o0 = coff(c), oj = coff(c - 2);
-            o.lvx_n = ld(Lvx, o1); o.bx_n = ld(BX, o1); o.fvx_n = ld(Fvx, o1);
-            o.lvz = ld(Lvz, o0); o.bz = ld(BZ, o0);
-            o.lpx = ld(Lpx, o0); o.lpz = ld(Lpz, o0);
+            o.lam = ld4(Lin, 4 * o1); o.bx_n = ld(BX, o1); o.fvx_n = ld(Fvx, o1);
+            o.bz = ld(BZ, o0);
             o.fvz = ld(Fvz, o0); o.fpx = ld(Fpx, o0); o.fpz = ld(Fpz, o0);
             o.kx = ld(KX, o0);
             o.kz = KZA ? 0.0f : ld(KZ, o0);
             o.gk = ldw(GK, o0);
-            o.lvz_j = ld(Lvz, oj); o.gbx_j = ldw(GBX, oj); o.gbz_j = ldw(GBZ, oj);
+            o.gbx_j = ldw(GBX, oj); o.gbz_j = ldw(GBZ, oj);
             o.res = -0.0f;
             if (tile_band) {
                 if (band_lane) o.res = rb[(size_t)r * nnx + clampc(c)];
@@ -239,19 +245,22 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             b = p1 * g;
         };
 
-        // vx-style windows: [k & 3] = column c-2+k.  p-style: [k & 3] = column c-3+k.
-        float LVX[4], FVX[4], AXP[4], BXP[4];
-        float AM[4], BM[4], AMZ[4], BMZ[4], PN[4];
+        // vx-style windows: [k & 3] = column c-2+k (LVX, LPX, LPZ, LVZ: the
+        // input adjoint state).  p-style: [k & 3] = column c-3+k (OPX, OPZ:
+        // the new L_px, L_pz, stored with phase B).
+        float LVX[4], LPX[4], LPZ[4], LVZ[4], FVX[4], AXP[4], BXP[4];
+        float AM[4], BM[4], AMZ[4], BMZ[4], PN[4], OPX[4], OPZ[4];
         #pragma unroll
         for (int k = 0; k < 4; k++) {
-            LVX[k] = FVX[k] = AXP[k] = BXP[k] = 0.0f;
-            AM[k] = BM[k] = AMZ[k] = BMZ[k] = PN[k] = 0.0f;
+            LVX[k] = LPX[k] = LPZ[k] = LVZ[k] = FVX[k] = AXP[k] = BXP[k] = 0.0f;
+            AM[k] = BM[k] = AMZ[k] = BMZ[k] = PN[k] = OPX[k] = OPZ[k] = 0.0f;
         }
         #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int col = xa - 3 + k;
             const unsigned o = coff(col);
-            LVX[k] = ld(Lvx, o);
+            const fquad lv = ld4(Lin, 4 * o);
+            LPX[k] = lv.x; LPZ[k] = lv.y; LVX[k] = lv.z; LVZ[k] = lv.w;
             FVX[k] = ld(Fvx, o);
             gx_terms(col, LVX[k], ld(BX, o), AXP[k], BXP[k]);
         }
@@ -268,7 +277,10 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             AdjCol &cur = q[u % PF];
 
             // ---- phase A at column c
-            const float lvx_n = keep(cur.lvx_n);
+            LPX[i3] = keep(cur.lam.x);
+            LPZ[i3] = keep(cur.lam.y);
+            LVZ[i3] = keep(cur.lam.w);
+            const float lvx_n = keep(cur.lam.z);
             LVX[i3] = lvx_n;
             FVX[i3] = keep(cur.fvx_n);
             if (FAST) {
@@ -279,13 +291,13 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
                 gx_terms(c + 1, lvx_n, cur.bx_n, AXP[i3], BXP[i3]);
             }
             const float ax = AXP[i1] - AXP[i2] + (BXP[i0] - BXP[i3]);
-            const float gz = (cur.bz * sbz) * cur.lvz;
+            const float gz = (cur.bz * sbz) * LVZ[i2];
             const float az = zp0 * gz, bzv = zp1 * gz;
             const float bz_u1 = lane_up1(bzv);
             const float azz = (lane_up1(az) - az) + (lane_up1(bz_u1) - lane_dn1(bzv));
             const float lp = ax + azz;
-            const float lpx1 = (cur.lpx + cur.res) + lp;
-            const float lpz1 = (cur.lpz + cur.res) + lp;
+            const float lpx1 = (LPX[i2] + cur.res) + lp;
+            const float lpz1 = (LPZ[i2] + cur.res) + lp;
             float xm0 = SAM2D_A0, xm1 = SAM2D_A1;
             if (!FAST) dminus_coef(c, nnx, xm0, xm1);
             // dJ/dkappa from the p-update needs Dx- vx[n-1] and Dz- vz[n-1];
@@ -305,9 +317,9 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
                 if (src_lane && c == src_x) g += lpx1 * srcg;
                 const unsigned id = (unsigned)c * (unsigned)nnz * 4u + rc4;
                 stf(GK, id, g);
-                stf(Opx, id, dax * lpx1);
-                stf(Opz, id, daz * lpz1);
             }
+            OPX[i3] = dax * lpx1;
+            OPZ[i3] = daz * lpz1;
             const float kz = KZA ? cur.kx : cur.kz;
             const float hx = (cur.kx * skx) * lpx1;
             const float hz = (kz * skz) * lpz1;
@@ -337,10 +349,10 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
                 const float dpx = fmaf(xp0, PN[i2] - pj, xp1 * (PN[i3] - PN[i0]));
                 const float dpz = fmaf(zp0, q_d1 - pj, zp1 * (q_d2 - q_u1));
                 const unsigned id = (unsigned)j * (unsigned)nnz * 4u + rc4;
-                stf(Ovx, id, fmaf(dbx, lvx_j, tx));
-                stf(Ovz, id, fmaf(dbz, cur.lvz_j, tz));
+                const float lvz_j = LVZ[i0];
+                st4(Lout, 4 * id, OPX[i1], OPZ[i1], fmaf(dbx, lvx_j, tx), fmaf(dbz, lvz_j, tz));
                 stf(GBX, id, cur.gbx_j + lvx_j * dpx * (dtdx * sbx));
-                stf(GBZ, id, cur.gbz_j + cur.lvz_j * dpz * (dtdz * sbz));
+                stf(GBZ, id, cur.gbz_j + lvz_j * dpz * (dtdz * sbz));
             }
             cur = load_col(c + PF);
         };
@@ -383,9 +395,9 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
 // Forward states: vx, vz after step n-2 in ring slot fa, after step n-1 in fb;
 // px, pz after step n-1 in fb, after step n in fc.
 struct AdjCol2 {
-    float lvx_n, bx_n, fvx_n;   // column c+1: L_vx, BX, fwd vx[n-1]
-    float lvz, bz;              // column c:   L_vz, BZ
-    float lpx, lpz;             //             L_px, L_pz
+    fquad lam;                  // column c+1: adjoint state (cell-interleaved)
+    float bx_n, fvx_n;          //             BX, fwd vx[n-1]
+    float bz;                   // column c:   BZ
     float fvz, fpx, fpz;        //             fwd vz[n-1], px[n], pz[n]
     float kx, kz, gk, res1;     //             KX, KZ, dJ/dkappa, residual of step n
     float gbx_j, gbz_j;         // column c-2: dJ/dbu2, dJ/dbu1
@@ -426,14 +438,8 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         const unsigned rc4 = rc * 4u;
         const size_t N = d.ncell;
 
-        gcchar *__restrict__ Lpx = (gcchar *)(d.lam + (4 * (size_t)lam_in + 0) * N);
-        gcchar *__restrict__ Lpz = (gcchar *)(d.lam + (4 * (size_t)lam_in + 1) * N);
-        gcchar *__restrict__ Lvx = (gcchar *)(d.lam + (4 * (size_t)lam_in + 2) * N);
-        gcchar *__restrict__ Lvz = (gcchar *)(d.lam + (4 * (size_t)lam_in + 3) * N);
-        gchar *__restrict__ Opx = (gchar *)(d.lam + (4 * (size_t)lam_out + 0) * N);
-        gchar *__restrict__ Opz = (gchar *)(d.lam + (4 * (size_t)lam_out + 1) * N);
-        gchar *__restrict__ Ovx = (gchar *)(d.lam + (4 * (size_t)lam_out + 2) * N);
-        gchar *__restrict__ Ovz = (gchar *)(d.lam + (4 * (size_t)lam_out + 3) * N);
+        gcchar *__restrict__ Lin = (gcchar *)(d.lam + 4 * (size_t)lam_in * N);
+        gchar *__restrict__ Lout = (gchar *)(d.lam + 4 * (size_t)lam_out * N);
         // psum bit 0 (slot fc) and bit 1 (slot fb): the slot holds p in its px
         // field and no pz; the pz loads then read the px field again.
         const bool ps_c = psum & 1, ps_b = (psum >> 1) & 1;
@@ -457,6 +463,15 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
 
         auto ld = [](gcchar *__restrict__ base, unsigned off) -> float {
             return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
+        };
+        // Adjoint state: byte offset 4x that of one field.
+        auto ld4 = [](gcchar *__restrict__ base, unsigned off) -> fquad {
+            return *(const SAM2D_GLOBAL fquad *__restrict__)(base + off);
+        };
+        auto st4 = [](gchar *__restrict__ base, unsigned off, float a, float b, float c, float e) {
+            fquad v;
+            v.x = a; v.y = b; v.z = c; v.w = e;
+            *(SAM2D_GLOBAL fquad *__restrict__)(base + off) = v;
         };
         auto ldw = [](gchar *__restrict__ base, unsigned off) -> float {
             return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
@@ -503,9 +518,8 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         auto load_col = [&](int c) -> AdjCol2 {
             AdjCol2 o;
             const unsigned o1 = coff(c + 1), o0 = coff(c), oj = coff(c - 2), o3 = coff(c - 3);
-            o.lvx_n = ld(Lvx, o1); o.bx_n = ld(BX, o1); o.fvx_n = ld(Fvx1, o1);
-            o.lvz = ld(Lvz, o0); o.bz = ld(BZ, o0);
-            o.lpx = ld(Lpx, o0); o.lpz = ld(Lpz, o0);
+            o.lam = ld4(Lin, 4 * o1); o.bx_n = ld(BX, o1); o.fvx_n = ld(Fvx1, o1);
+            o.bz = ld(BZ, o0);
             o.fvz = ld(Fvz1, o0); o.fpx = ld(Fpx1, o0); o.fpz = ld(Fpz1, o0);
             o.kx = ld(KX, o0);
             o.kz = KZA ? 0.0f : ld(KZ, o0);
@@ -529,24 +543,27 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         //   LVZ0 BZW KXW KZW LPX1 LPZ1 GKW H1 HZ1 PN1   c-3 .. c
         //   G2 LVX1 LVZ1 GBXW GBZW FVX2           c-5 .. c-2
         //   H2 HZ2 PN2                            c-6 .. c-3
-        float G1[4], LVX0[4], FVX1[4], BXW[4];
+        //   LPXn LPZn LVZn (input L_px, L_pz, L_vz)  c-2 .. c+1
+        //   OPA OPZA (new L_px, L_pz of step n-1)   c-6 .. c-3, stored with B2
+        float G1[4], LVX0[4], FVX1[4], BXW[4], LPXn[4], LPZn[4], LVZn[4];
         float LVZ0[4], BZW[4], KXW[4], KZW[4], LPX1[4], LPZ1[4], GKW[4], H1[4], HZ1[4], PN1[4];
         float G2[4], LVX1[4], LVZ1[4], GBXW[4], GBZW[4], FVX2[4];
-        float H2[4], HZ2[4], PN2[4];
+        float H2[4], HZ2[4], PN2[4], OPA[4], OPZA[4];
         #pragma unroll
         for (int k = 0; k < 4; k++) {
-            G1[k] = LVX0[k] = FVX1[k] = BXW[k] = 0.0f;
+            G1[k] = LVX0[k] = FVX1[k] = BXW[k] = LPXn[k] = LPZn[k] = LVZn[k] = 0.0f;
             LVZ0[k] = BZW[k] = KXW[k] = KZW[k] = LPX1[k] = LPZ1[k] = GKW[k] = 0.0f;
             H1[k] = HZ1[k] = PN1[k] = 0.0f;
             G2[k] = LVX1[k] = LVZ1[k] = GBXW[k] = GBZW[k] = FVX2[k] = 0.0f;
-            H2[k] = HZ2[k] = PN2[k] = 0.0f;
+            H2[k] = HZ2[k] = PN2[k] = OPA[k] = OPZA[k] = 0.0f;
         }
         const int c_first = xa - 4;
         #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int col = c_first - 2 + k;
             const unsigned o = coff(col);
-            LVX0[k] = ld(Lvx, o);
+            const fquad lv = ld4(Lin, 4 * o);
+            LPXn[k] = lv.x; LPZn[k] = lv.y; LVX0[k] = lv.z; LVZn[k] = lv.w;
             FVX1[k] = ld(Fvx1, o);
             BXW[k] = ld(BX, o);
             G1[k] = (BXW[k] * prx[3 * nnx + clampc(col)]) * LVX0[k];
@@ -565,7 +582,10 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
 
             // ---- A1: step n, phase A at column c
             {
-                const float lvx_n = keep(cur.lvx_n);
+                LPXn[i3] = keep(cur.lam.x);
+                LPZn[i3] = keep(cur.lam.y);
+                LVZn[i3] = keep(cur.lam.w);
+                const float lvx_n = keep(cur.lam.z);
                 LVX0[i3] = lvx_n;
                 FVX1[i3] = keep(cur.fvx_n);
                 const float bxn = keep(cur.bx_n);
@@ -579,7 +599,7 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                     ax = p0_at(c - 1) * G1[i1] - p0_at(c) * G1[i2] +
                          (p1_at(c - 2) * G1[i0] - p1_at(c + 1) * G1[i3]);
                 }
-                const float lvz = keep(cur.lvz);
+                const float lvz = LVZn[i2];
                 LVZ0[i3] = lvz;
                 const float bz = keep(cur.bz);
                 BZW[i3] = bz;
@@ -588,8 +608,8 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                 const float bz_u1 = lane_up1(bzv);
                 const float azz = (lane_up1(az) - az) + (lane_up1(bz_u1) - lane_dn1(bzv));
                 const float lp = ax + azz;
-                const float lpx1 = (cur.lpx + cur.res1) + lp;
-                const float lpz1 = (cur.lpz + cur.res1) + lp;
+                const float lpx1 = (LPXn[i2] + cur.res1) + lp;
+                const float lpz1 = (LPZn[i2] + cur.res1) + lp;
                 float xm0 = SAM2D_A0, xm1 = SAM2D_A1, skx = 1.0f, dax = 1.0f;
                 if (!FAST) {
                     dminus_coef(c, nnx, xm0, xm1);
@@ -684,9 +704,9 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                     if (src_lane && c2 == src_x2) g += lpx2 * srcg2;
                     const unsigned id = (unsigned)c2 * (unsigned)nnz * 4u + rc4;
                     stf(GK, id, g);
-                    stf(Opx, id, dax * lpx2);
-                    stf(Opz, id, daz * lpz2);
                 }
+                OPA[i3] = dax * lpx2;
+                OPZA[i3] = daz * lpz2;
                 const float kz = KZA ? KXW[i0] : KZW[i0];
                 H2[i3] = (KXW[i0] * skx) * lpx2;
                 HZ2[i3] = (kz * skz) * lpz2;
@@ -721,8 +741,7 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                     const float dpx = fmaf(xp0, PN2[i2] - pj, xp1 * (PN2[i3] - PN2[i0]));
                     const float dpz = fmaf(zp0, q_d1 - pj, zp1 * (q_d2 - q_u1));
                     const unsigned id = (unsigned)j2 * (unsigned)nnz * 4u + rc4;
-                    stf(Ovx, id, fmaf(dbx, lvx_j, tx));
-                    stf(Ovz, id, fmaf(dbz, lvz_j, tz));
+                    st4(Lout, 4 * id, OPA[i1], OPZA[i1], fmaf(dbx, lvx_j, tx), fmaf(dbz, lvz_j, tz));
                     stf(GBX, id, GBXW[i0] + lvx_j * dpx * (dtdx * sbx));
                     stf(GBZ, id, GBZW[i0] + lvz_j * dpz * (dtdz * sbz));
                 }

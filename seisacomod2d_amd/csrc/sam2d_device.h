@@ -73,7 +73,8 @@ struct AdjDev {
     const float *KX, *KZ, *BX, *BZ;
     const float *prof_x, *prof_z;
     const float *fwd;      // forward ring: slots of 4 fields (px, pz, vx, vz)
-    float *lam;            // adjoint state: 2 slots of 4 fields
+    float *lam;            // adjoint state: 2 slots of 4*ncell floats, cell-interleaved
+                           // (L_px, L_pz, L_vx, L_vz of cell i at 4i .. 4i+3)
     size_t ncell;
     float *gk;             // dJ/dkappa      (padded grid)
     float *gbx;            // dJ/dbu2        (x-staggered buoyancy)
