@@ -158,8 +158,10 @@ int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
  * order, s = 2..64 = with stride s, for the multi-step kernels; set by
  * sam2d_autotune, default 0; results are the same bits),
  * "adjoint_tile_order" (the same for the adjoint kernels, default 7),
- * "chunk" (columns each multi-step tile marches, 0 = the variant's own; set
- * by sam2d_autotune; results are the same bits). */
+ * "chunk" (columns each multi-step tile marches, 0 = the variant's own
+ * rounded up so that the march, chunk + 6T - 3 columns for T steps per
+ * launch, is whole groups of four columns; set by sam2d_autotune; results are
+ * the same bits). */
 int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value);
 /* Current value of an option above (after sam2d_autotune: "tile_order"). */
 int sam2d_get_option(sam2d_ctx *ctx, const char *key, int *value);

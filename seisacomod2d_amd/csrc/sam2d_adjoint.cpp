@@ -249,6 +249,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     if (two_step) {
         int w2, c2;
         variant_shape(variant, &w2, &c2);
+        c2 = march_aligned(c2, 9);
         std::vector<AdjDev> desc2(desc);
         for (auto &dd : desc2) {
             tile_shape(dd.nnz, dd.nnx, w2, c2, OWN_Z2, dd.ngroups, dd.nchunks);

@@ -143,6 +143,8 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         if (zb >= nnz) continue;                      // wave-uniform
         const int xa = chunk * d.chunk;
         const int xb = min(xa + d.chunk, nnx);
+        // Columns up to c_last: the stores need j = c - 2 in [xa, xb).
+        const int c_last = xb + 1;
 
         const int r = zb - HALO_T + lane;
         const bool rown = (lane >= HALO_T) && (lane < HALO_T + OWN_Z) && (r < nnz);
@@ -354,7 +356,7 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
                 stf(GBX, id, cur.gbx_j + lvx_j * dpx * (dtdx * sbx));
                 stf(GBZ, id, cur.gbz_j + lvz_j * dpz * (dtdz * sbz));
             }
-            cur = load_col(c + PF);
+            cur = load_col(min(c + PF, c_last));   // past c_last: a cache hit
         };
 
         using TT = std::integral_constant<bool, true>;
@@ -363,21 +365,25 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         using U1 = std::integral_constant<int, 1>;
         using U2 = std::integral_constant<int, 2>;
         using U3 = std::integral_constant<int, 3>;
-        // Slow groups, the fast range, slow groups.
-        const int c_last = xb + 1;
+        // Slow groups, the fast range, slow groups; the march ends at c_last.
         int c0 = xa - 1;
-        for (; c0 <= c_last && c0 < fast_lo; c0 += 4) {
+        for (; c0 + 3 <= c_last && c0 < fast_lo; c0 += 4) {
             column(FF(), U0(), c0); column(FF(), U1(), c0);
             column(FF(), U2(), c0); column(FF(), U3(), c0);
         }
-        const int c_fast = min(fast_hi, c_last);
+        const int c_fast = min(fast_hi, c_last - 3);
         for (; c0 <= c_fast; c0 += 4) {
             column(TT(), U0(), c0); column(TT(), U1(), c0);
             column(TT(), U2(), c0); column(TT(), U3(), c0);
         }
         for (; c0 <= c_last; c0 += 4) {
-            column(FF(), U0(), c0); column(FF(), U1(), c0);
-            column(FF(), U2(), c0); column(FF(), U3(), c0);
+            column(FF(), U0(), c0);
+            if (c0 + 1 > c_last) break;
+            column(FF(), U1(), c0);
+            if (c0 + 2 > c_last) break;
+            column(FF(), U2(), c0);
+            if (c0 + 3 > c_last) break;
+            column(FF(), U3(), c0);
         }
     }
 }
@@ -431,6 +437,8 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         if (zb >= nnz) continue;                      // wave-uniform
         const int xa = chunk * d.chunk;
         const int xb = min(xa + d.chunk, nnx);
+        // Columns up to c_last: the B-phase stores j = c - 5 in [xa, xb).
+        const int c_last = xb + 4;
 
         const int r = zb - HALO_T2 + lane;
         const bool rown = (lane >= HALO_T2) && (lane < HALO_T2 + OWN_Z2) && (r < nnz);
@@ -746,7 +754,7 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                     stf(GBZ, id, GBZW[i0] + lvz_j * dpz * (dtdz * sbz));
                 }
             }
-            cur = load_col(c + PF);
+            cur = load_col(min(c + PF, c_last));   // past c_last: a cache hit
         };
 
         using TT = std::integral_constant<bool, true>;
@@ -755,20 +763,24 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         using U1 = std::integral_constant<int, 1>;
         using U2 = std::integral_constant<int, 2>;
         using U3 = std::integral_constant<int, 3>;
-        const int c_last = xb + 4;
         int c0 = c_first;
-        for (; c0 <= c_last && c0 < fast_lo; c0 += 4) {
+        for (; c0 + 3 <= c_last && c0 < fast_lo; c0 += 4) {
             column(FF(), U0(), c0); column(FF(), U1(), c0);
             column(FF(), U2(), c0); column(FF(), U3(), c0);
         }
-        const int c_fast = min(fast_hi, c_last);
+        const int c_fast = min(fast_hi, c_last - 3);
         for (; c0 <= c_fast; c0 += 4) {
             column(TT(), U0(), c0); column(TT(), U1(), c0);
             column(TT(), U2(), c0); column(TT(), U3(), c0);
         }
         for (; c0 <= c_last; c0 += 4) {
-            column(FF(), U0(), c0); column(FF(), U1(), c0);
-            column(FF(), U2(), c0); column(FF(), U3(), c0);
+            column(FF(), U0(), c0);
+            if (c0 + 1 > c_last) break;
+            column(FF(), U1(), c0);
+            if (c0 + 2 > c_last) break;
+            column(FF(), U2(), c0);
+            if (c0 + 3 > c_last) break;
+            column(FF(), U3(), c0);
         }
     }
 }

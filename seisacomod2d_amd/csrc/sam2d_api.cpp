@@ -218,7 +218,9 @@ int tile_chunk(const sam2d_ctx *ctx, int variant)
 {
     int waves, chunk;
     variant_shape(variant, &waves, &chunk);
-    return ctx->chunk > 0 && variant_steps(variant) > 1 ? ctx->chunk : chunk;
+    const int steps = variant_steps(variant);
+    if (steps == 1) return chunk;
+    return ctx->chunk > 0 ? ctx->chunk : march_aligned(chunk, 6 * steps - 3);
 }
 
 // Make `plans` resident: buffers, descriptors, zeroed state and seismograms.
@@ -798,7 +800,8 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
         if (ms < best_ms) { best_ms = ms; best = v; }
     }
     // Second pass, for the three fastest multi-step variants: the march
-    // length (the variant's chunk scaled by 1/2 .. 3/2, multiples of 4)
+    // length (the variant's chunk scaled by 1/2 .. 3/2, rounded to whole
+    // four-column groups of the march)
     // jointly with the chunk-column order (stride 1 = in order, 2..9), each
     // combination timed twice over four times the steps.  The four fastest
     // combinations are then timed three times over twelve times the steps
@@ -840,13 +843,14 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
             variant_shape(v, &w0, &c0);
             std::vector<int> chunks;
             for (int eighths : {4, 5, 6, 7, 8, 9, 10, 12}) {
-                const int c = std::max(16, (c0 * eighths / 8 + 2) / 4 * 4);
+                const int c = march_aligned(std::max(16, (c0 * eighths / 8 + 2) / 4 * 4),
+                                            6 * variant_steps(v) - 3);
                 if (std::find(chunks.begin(), chunks.end(), c) == chunks.end())
                     chunks.push_back(c);
             }
             for (int c : chunks)
                 for (int xo = 1; xo <= 9; xo++) {
-                    Combo cb{0.0, v, c == c0 ? 0 : c, xo};
+                    Combo cb{0.0, v, c, xo};
                     SAM2D_TRY(time_combo(cb, 4 * steps, 2, &cb.ms));
                     combos.push_back(cb);
                 }

@@ -94,6 +94,14 @@ struct TileOrder {
     int ngroups;
     int xorder;
 };
+// March length rounded up so that a march over chunk + extra columns is
+// whole groups of four: the kernels march four columns per group, and a
+// partial last group runs on the per-column (edge) path.  extra: 6T - 3 for
+// the T-step forward kernels, 9 for the two-step adjoint.
+inline int march_aligned(int chunk, int extra)
+{
+    return chunk + (4 - (chunk + extra) % 4) % 4;
+}
 // Blocks per XCD range for a launch of ntiles tiles: a multiple of ngroups
 // when the columns are reordered (the tiles must then be whole columns of
 // ngroups); otherwise the order falls back to the plain one.

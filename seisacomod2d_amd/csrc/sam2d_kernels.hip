@@ -201,12 +201,13 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         #pragma unroll
         for (int u = 0; u < PF; u++) q[u] = load_col(xa - 1 + u);
 
-        // The march may run up to three columns past xb+1: those columns store
-        // nothing (stores need c < xb, j < xb).
+        // The march ends at column xb+1 (stores need c < xb, j < xb); the
+        // prefetches past it re-read column xb+1, a cache hit.
         for (int c0 = xa - 1; c0 <= xb + 1; c0 += 4) {
             #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int c = c0 + u;
+                if (c > xb + 1) break;
                 ColOps &cur = q[u % PF];
                 // ---- p-phase at column c (cpu_pml_p, :335-351; source, :135-140)
                 if constexpr (AOS) {
@@ -265,7 +266,7 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
                         }
                     }
                 }
-                cur = load_col(c + PF);
+                cur = load_col(min(c + PF, xb + 1));
             }
         }
     }

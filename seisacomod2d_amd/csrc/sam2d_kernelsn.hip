@@ -213,7 +213,11 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                 VX[t][k] = VZ[t][k] = BXw[t][k] = BZw[t][k] = 0.0f;
             }
         }
+        // Columns c_first .. c_last: the last step stores jv = c - 3T + 1 in
+        // [xa, xb).  Prefetches past c_last re-read column c_last (a cache
+        // hit) and the march ends at c_last exactly.
         const int c_first = xa - (3 * T - 2);
+        const int c_last = xb + 3 * T - 2;
         if constexpr (AOS) {
             #pragma unroll
             for (int k = 0; k < 3; k++) {
@@ -357,7 +361,7 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                     }
                 }
             }
-            cur = load_col(c + PF);
+            cur = load_col(min(c + PF, c_last));
         };
 
         using TT = std::integral_constant<bool, true>;
@@ -366,20 +370,24 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
         using U1 = std::integral_constant<int, 1>;
         using U2 = std::integral_constant<int, 2>;
         using U3 = std::integral_constant<int, 3>;
-        const int c_last = xb + 3 * T - 2;
         int c0 = c_first;
-        for (; c0 <= c_last && c0 < fast_lo; c0 += 4) {
+        for (; c0 + 3 <= c_last && c0 < fast_lo; c0 += 4) {
             column(FF(), U0(), c0); column(FF(), U1(), c0);
             column(FF(), U2(), c0); column(FF(), U3(), c0);
         }
-        const int c_fast = min(fast_hi, c_last);
+        const int c_fast = min(fast_hi, c_last - 3);
         for (; c0 <= c_fast; c0 += 4) {
             column(TT(), U0(), c0); column(TT(), U1(), c0);
             column(TT(), U2(), c0); column(TT(), U3(), c0);
         }
         for (; c0 <= c_last; c0 += 4) {
-            column(FF(), U0(), c0); column(FF(), U1(), c0);
-            column(FF(), U2(), c0); column(FF(), U3(), c0);
+            column(FF(), U0(), c0);
+            if (c0 + 1 > c_last) break;
+            column(FF(), U1(), c0);
+            if (c0 + 2 > c_last) break;
+            column(FF(), U2(), c0);
+            if (c0 + 3 > c_last) break;
+            column(FF(), U3(), c0);
         }
     }
 }

@@ -238,7 +238,9 @@ def test_tile_order_bitwise():
         for name in ("threestep_waves1_chunk32", "twostep_waves2_chunk32_pf4",
                      "fivestep_waves1_chunk32"):
             m.set_option("variant", names.index(name))
-            for xo, chunk in ((1, 0), (2, 20), (3, 0), (4, 44), (7, 0), (64, 8)):
+            # March tails of 0 .. 3 columns past the last whole group of four.
+            for xo, chunk in ((1, 0), (2, 20), (3, 0), (4, 44), (5, 42), (6, 43), (7, 0),
+                              (64, 8)):
                 m.set_option("tile_order", xo)
                 m.set_option("chunk", chunk)
                 out = m.forward_batched(shots, wins)

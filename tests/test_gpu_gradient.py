@@ -118,7 +118,7 @@ def test_multistep_forward_sweep_bitwise():
         # states; both adjoint kernels read those slots.  The chunk-column
         # order of the forward sweep and the recomputation changes nothing.
         m.set_option("variant", names.index("threestep_waves2_chunk64"))
-        for steps, xo, chunk in ((1, 0, 0), (2, 3, 40), (1, 5, 24)):
+        for steps, xo, chunk in ((1, 0, 0), (2, 3, 40), (1, 5, 24), (2, 6, 43), (2, 2, 42)):
             m.set_option("adjoint_steps", steps)
             m.set_option("tile_order", xo)
             m.set_option("chunk", chunk)
