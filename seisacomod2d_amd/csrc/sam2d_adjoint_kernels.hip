@@ -172,7 +172,7 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         gchar *__restrict__ GK = (gchar *)d.gk;
         gchar *__restrict__ GBX = (gchar *)d.gbx;
         gchar *__restrict__ GBZ = (gchar *)d.gbz;
-        const SAM2D_GLOBAL float *__restrict__ prx = gp(d.prof_x);
+        const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);
         const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
 
         auto ld = [](gcchar *__restrict__ base, unsigned off) -> float {
@@ -466,7 +466,7 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         gchar *__restrict__ GK = (gchar *)d.gk;
         gchar *__restrict__ GBX = (gchar *)d.gbx;
         gchar *__restrict__ GBZ = (gchar *)d.gbz;
-        const SAM2D_GLOBAL float *__restrict__ prx = gp(d.prof_x);
+        const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);
         const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
 
         auto ld = [](gcchar *__restrict__ base, unsigned off) -> float {

@@ -38,6 +38,12 @@ __device__ __forceinline__ void dplus_coef(int i, int n, float &c0, float &c1)
 typedef float fquad __attribute__((ext_vector_type(4)));
 template <class T>
 __device__ __forceinline__ SAM2D_GLOBAL T *gp(T *p) { return (SAM2D_GLOBAL T *)p; }
+// Read-only tables indexed by a wave-uniform column (the x-profiles of the
+// sponge): in the constant address space the compiler reads them with scalar
+// loads, off the vector memory path that streams the state.
+#define SAM2D_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const SAM2D_CONST T *cp4(const T *p) { return (const SAM2D_CONST T *)p; }
 
 __device__ __forceinline__ float shfl_up(float v, int d) { return __shfl_up(v, d, 64); }
 __device__ __forceinline__ float shfl_dn(float v, int d) { return __shfl_down(v, d, 64); }

@@ -144,7 +144,7 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         const SAM2D_GLOBAL float *__restrict__ KZ = gp(d.KZ);
         const SAM2D_GLOBAL float *__restrict__ BX = gp(d.BX);
         const SAM2D_GLOBAL float *__restrict__ BZ = gp(d.BZ);
-        const SAM2D_GLOBAL float *__restrict__ prx = gp(d.prof_x);   // [4][nnx]
+        const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);   // [4][nnx]
         const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);   // [4][nnz]
 
         // Per-row constants: sponge profiles and z-operator coefficients.

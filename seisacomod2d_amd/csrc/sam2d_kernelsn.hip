@@ -118,7 +118,7 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
         gcchar *__restrict__ KZ = (gcchar *)d.KZ;
         gcchar *__restrict__ BX = (gcchar *)d.BX;
         gcchar *__restrict__ BZ = (gcchar *)d.BZ;
-        const SAM2D_GLOBAL float *__restrict__ prx = gp(d.prof_x);
+        const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);
         const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
         // Intermediate-state slots (RING only).
         gchar *__restrict__ mid[T][4];
