@@ -799,7 +799,7 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
         ctx->tune_ms[v] = ms / steps;
         if (ms < best_ms) { best_ms = ms; best = v; }
     }
-    // Second pass, for the three fastest multi-step variants: the march
+    // Second pass, for the six fastest multi-step variants: the march
     // length (the variant's chunk scaled by 1/2 .. 3/2, rounded to whole
     // four-column groups of the march)
     // jointly with the chunk-column order (stride 1 = in order, 2..9), each
@@ -836,7 +836,10 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
     std::sort(multi.begin(), multi.end(),
               [&](int a, int b) { return ctx->tune_ms[a] < ctx->tune_ms[b]; });
     if (variant_steps(best) > 1 && !multi.empty()) {
-        if (multi.size() > 3) multi.resize(3);
+        // At its own march length a variant can rank several places below
+        // where its best length puts it (C1: fifth of the first pass, first of
+        // the second, 251 against 272 us per launch).
+        if (multi.size() > 6) multi.resize(6);
         std::vector<Combo> combos;
         for (int v : multi) {
             int w0, c0;
