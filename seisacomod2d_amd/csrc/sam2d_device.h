@@ -53,8 +53,9 @@ __host__ __device__ inline float *slot_field(const ShotDev &d, int slot, int f)
 // px, pz, vx, vz of cell i at [4i .. 4i+3].  Per column a wave then reads
 // and writes one contiguous kilobyte of state instead of four 256-byte runs
 // ncell*4 bytes apart, which the memory system streams about 18 % faster for
-// the multi-step kernels' tile pattern on MI355X.  The gradient keeps one
-// array per field (its adjoint reads single fields of the forward states).
+// the multi-step kernels' tile pattern on MI355X.  The gradient's forward sweep
+// and checkpoints use it too; its recomputation ring keeps one array per field
+// (the adjoint reads single fields of the forward states).
 __host__ __device__ inline float *slot_cells(const ShotDev &d, int slot)
 {
     return d.state + (size_t)slot * 4 * d.ncell;
