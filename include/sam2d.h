@@ -92,9 +92,16 @@ typedef struct sam2d_stats {
 typedef struct sam2d_ctx sam2d_ctx;
 
 /* Replaces modelling_module's setup (cpumodelling.cpp:168-281): validates the
- * grid (fdop == 4, fd_dt divides real_dt, model values positive), uploads the
- * model and builds the FP32 coefficient fields on the device.
- * Does not run the CFL / dispersion check; see sam2d_check_stability. */
+ * grid (sizes and spacings, fdop == 4, fd_dt divides real_dt, a padded grid
+ * below 2^31 cells), uploads the model and builds the FP32 coefficient fields
+ * on the device.  Model values are taken as they are, as the reference takes
+ * them (a density <= 0 gives the buoyancy 1e10, cpu_modelling_kernels.cpp:64-121).
+ * Does not run the CFL / dispersion check; see sam2d_check_stability.
+ * Size limit of the multi-step kernels: their tiles address a shot's state
+ * with 32-bit byte offsets, so a padded grid (or window) of 2^28 cells or more
+ * runs on the single-step kernels only; sam2d_autotune skips the multi-step
+ * variants there, and choosing one with the "variant" option makes the
+ * modelling calls return SAM2D_ERR_PARAM. */
 int sam2d_create(const sam2d_grid *grid, const float *vp, const float *rho,
                  sam2d_ctx **out);
 
@@ -146,8 +153,13 @@ int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
                    float *seis, float *g_vp, float *g_rho);
 
 /* Tuning and options.  Keys: "graph" (0/1, default 1), "variant" (-1 = auto,
- * else a kernel variant index), "reset_quirk" (1 = zero only the first nnz
- * cells between shots, cpumodelling.cpp:343-348; default 0 = fresh state),
+ * else a kernel variant index), "reset_quirk" (1 = each shot of the per-shot
+ * host-output loop, sam2d_forward, starts from the final state of the shot
+ * modelled before it in that mode on this context, in the same call or an
+ * earlier one, with only the first nnz cells cleared, as the reference's
+ * modelling_module does between its shots, cpumodelling.cpp:343-348; the first
+ * shot after the option is set, and the first after any other modelling call,
+ * starts fresh; default 0 = fresh state for every shot),
  * "max_batch" (shots resident at once in batched and gradient calls, 0 = all
  * or, for gradients, as many as fit "hbm_budget_gb" (default 64 GiB of device
  * memory for ring, checkpoints and adjoint state)), "adjoint_steps" (backward

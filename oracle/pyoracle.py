@@ -213,9 +213,12 @@ def functional(P, vp, rho, src_xz, rec_xz_list, adj_list):
                             _ptr(nrec, ctypes.c_int), _ptr(rec), _ptr(adj))
 
 
-def gradient(P, vp, rho, src_xz, rec_xz_list, adj_list, padded=False):
+def gradient(P, vp, rho, src_xz, rec_xz_list, adj_list, padded=False, ckpt_k=0):
     """FP64 dJ/dvp, dJ/drho (nx, nz) of J = sum adj * p; with padded=True also
-    the padded-grid dJ/dkappa, dJ/dbu1, dJ/dbu2."""
+    the padded-grid dJ/dkappa, dJ/dbu1, dJ/dbu2.  ckpt_k > 0 checkpoints the
+    forward state every ckpt_k steps and recomputes the segments (memory
+    about (4 fd_nt / ckpt_k + 3 ckpt_k) N doubles instead of 3 N fd_nt); the
+    result is the same bits."""
     fd_nt, real_nt, dtf, nts, nnx, nnz = shape(P)
     vp, rho = _f32(vp), _f32(rho)
     src, nrec, rec, adj = _geom(src_xz, rec_xz_list, adj_list)
@@ -224,13 +227,14 @@ def gradient(P, vp, rho, src_xz, rec_xz_list, adj_list, padded=False):
     gk, gb1, gb2 = (np.zeros((nnx, nnz), np.float64) for _ in range(3))
     L = lib()
     dp = ctypes.POINTER(ctypes.c_double)
-    L.orc_gradient.argtypes = [ctypes.POINTER(OrcParams)] + [ctypes.POINTER(ctypes.c_float)] * 2 + [
+    L.orc_gradient_k.argtypes = [ctypes.POINTER(OrcParams)] + [ctypes.POINTER(ctypes.c_float)] * 2 + [
         ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int),
-        ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)] + [dp] * 5
-    rc = L.orc_gradient(ctypes.byref(P), _ptr(vp), _ptr(rho), src.shape[0], _ptr(src),
-                        _ptr(nrec, ctypes.c_int), _ptr(rec), _ptr(adj), _ptr(gvp, ctypes.c_double),
-                        _ptr(grho, ctypes.c_double), _ptr(gk, ctypes.c_double),
-                        _ptr(gb1, ctypes.c_double), _ptr(gb2, ctypes.c_double))
+        ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int] + [dp] * 5
+    rc = L.orc_gradient_k(ctypes.byref(P), _ptr(vp), _ptr(rho), src.shape[0], _ptr(src),
+                          _ptr(nrec, ctypes.c_int), _ptr(rec), _ptr(adj), int(ckpt_k),
+                          _ptr(gvp, ctypes.c_double), _ptr(grho, ctypes.c_double),
+                          _ptr(gk, ctypes.c_double), _ptr(gb1, ctypes.c_double),
+                          _ptr(gb2, ctypes.c_double))
     if rc != 0:
         raise RuntimeError(f"orc_gradient failed: {rc}")
     if padded:
@@ -241,3 +245,12 @@ def gradient(P, vp, rho, src_xz, rec_xz_list, adj_list, padded=False):
 def set_f64_buoyancy(on):
     """Test-only: evaluate 1/rho and the staggered buoyancy averages in FP64."""
     lib().orc_set_f64_buoyancy(ctypes.c_int(1 if on else 0))
+
+
+def set_reset_plant(cells=-1, fields=31):
+    """Test-only: what the reset between shots clears in reset_quirk mode
+    (cells < 0: the reference's first nnz cells; fields: mask of p, px, pz,
+    vx, vz), to plant a slip that the reference's two-shot output rejects."""
+    L = lib()
+    L.orc_set_reset_plant.argtypes = [ctypes.c_long, ctypes.c_int]
+    L.orc_set_reset_plant(int(cells), int(fields))

@@ -427,6 +427,18 @@ void orc_setup(const orc_params *P, const float *vp, const float *rho,
     orc_model_free(&m);
 }
 
+/* Test-only: change what the reset between shots clears in reset_quirk
+ * mode, so the tests can plant a slip and see the reference's two-shot
+ * output reject it.  cells < 0 restores the reference's nnz cells; fields
+ * is a mask of p, px, pz, vx, vz (bits 0..4; 31 = all, the reference). */
+static long orc_reset_cells = -1;
+static int orc_reset_fields = 31;
+void orc_set_reset_plant(long cells, int fields)
+{
+    orc_reset_cells = cells;
+    orc_reset_fields = fields;
+}
+
 /* ------------------------------------------------------------ forward */
 /* modelling_module() shot and time loops, cpumodelling.cpp:326-380, plus
  * cpu_record (:24-31, sampling p after step kt when (kt+1)%dt_factor==0) and
@@ -451,10 +463,16 @@ int orc_forward(const orc_params *P, const float *vp, const float *rho,
         long *gxz = (long *)malloc(sizeof(long) * (ng > 0 ? ng : 1));
         for (int g = 0; g < ng; g++)
             gxz[g] = orc_cell(&m, rec_xz[2 * (rec_off + g)], rec_xz[2 * (rec_off + g) + 1]);
-        long clear = P->reset_quirk ? m.nnz : m.N;
-        if (is == 0) clear = m.N;
+        long clear = P->reset_quirk ? (orc_reset_cells >= 0 ? orc_reset_cells : m.nnz) : m.N;
+        int fields = P->reset_quirk ? orc_reset_fields : 31;
+        if (is == 0) { clear = m.N; fields = 31; }
+        if (clear > m.N) clear = m.N;
         for (long i = 0; i < clear; i++) {
-            s.p[i] = 0.0; s.px[i] = 0.0; s.pz[i] = 0.0; s.vx[i] = 0.0; s.vz[i] = 0.0;
+            if (fields & 1) s.p[i] = 0.0;
+            if (fields & 2) s.px[i] = 0.0;
+            if (fields & 4) s.pz[i] = 0.0;
+            if (fields & 8) s.vx[i] = 0.0;
+            if (fields & 16) s.vz[i] = 0.0;
         }
         float *out = seis + seis_off;
         memset(out, 0, sizeof(float) * (size_t)ng * m.real_nt);
@@ -597,26 +615,39 @@ double orc_functional(const orc_params *P, const float *vp, const float *rho, in
 
 /* dJ/dvp and dJ/drho on the model grid (x-major nx*nz, FP64), plus the
  * intermediate gradients on the padded grid when the pointers are given:
- * g_kappa (dJ/dkappa), g_bu1, g_bu2 (dJ/d of the staggered buoyancies). */
-int orc_gradient(const orc_params *P, const float *vp, const float *rho, int ns,
-                 const float *src_xz, const int *nrec, const float *rec_xz, const float *adj,
-                 double *g_vp, double *g_rho, double *gk_out, double *gb1_out, double *gb2_out)
+ * g_kappa (dJ/dkappa), g_bu1, g_bu2 (dJ/d of the staggered buoyancies).
+ *
+ * ckpt_k > 0 checkpoints the forward state (px, pz, vx, vz) every ckpt_k
+ * steps and recomputes each segment of ckpt_k steps before its adjoint
+ * steps, so memory is 4 N doubles per checkpoint plus 3 N ckpt_k for the
+ * segment; ckpt_k <= 0 (or >= fd_nt) keeps every state of the record
+ * (3 N fd_nt doubles).  The recomputed states are the same bits as the
+ * first sweep's (same steps, same order), so the result does not depend on
+ * ckpt_k. */
+int orc_gradient_k(const orc_params *P, const float *vp, const float *rho, int ns,
+                   const float *src_xz, const int *nrec, const float *rec_xz, const float *adj,
+                   int ckpt_k, double *g_vp, double *g_rho, double *gk_out, double *gb1_out,
+                   double *gb2_out)
 {
     orc_model m;
     orc_model_init(&m, P->nx, P->nz, P->dx, P->dz, vp, rho, P->rec_len, P->fd_dt,
                    P->real_dt, P->fm, P->npml);
     const int nnz = m.nnz, nnx = m.nnx, nt = m.fd_nt;
     const long N = m.N;
+    const int K = (ckpt_k <= 0 || ckpt_k > nt) ? nt : ckpt_k;
+    const int nseg = (nt + K - 1) / K;
     double *gk = (double *)calloc(N, 8), *gb1 = (double *)calloc(N, 8), *gb2 = (double *)calloc(N, 8);
-    /* stored forward fields: vx, vz before step n and p after the p-phase of n */
-    double *Vx = (double *)malloc(sizeof(double) * N * nt);
-    double *Vz = (double *)malloc(sizeof(double) * N * nt);
-    double *Pn = (double *)malloc(sizeof(double) * N * nt);
+    /* segment ring: vx, vz before step n and p after the p-phase of n */
+    double *Vx = (double *)malloc(sizeof(double) * N * K);
+    double *Vz = (double *)malloc(sizeof(double) * N * K);
+    double *Pn = (double *)malloc(sizeof(double) * N * K);
+    /* checkpoint c: px, pz, vx, vz entering step c*K (c = 0 is the zero state) */
+    double *ck = (double *)calloc((size_t)4 * N * nseg, 8);
     double *lpx = (double *)calloc(N, 8), *lpz = (double *)calloc(N, 8);
     double *lvx = (double *)calloc(N, 8), *lvz = (double *)calloc(N, 8);
     double *lp = (double *)calloc(N, 8), *t1 = (double *)calloc(N, 8), *t2 = (double *)calloc(N, 8);
     double *nvx = (double *)calloc(N, 8), *nvz = (double *)calloc(N, 8);
-    if (!Vx || !Vz || !Pn) return -30;
+    if (!Vx || !Vz || !Pn || !ck) return -30;
     orc_state s;
     orc_state_alloc(&s, m.N);
     long rec_off = 0, a_off = 0;
@@ -626,54 +657,80 @@ int orc_gradient(const orc_params *P, const float *vp, const float *rho, int ns,
         long *gc = (long *)malloc(sizeof(long) * (ng > 0 ? ng : 1));
         for (int g = 0; g < ng; g++)
             gc[g] = orc_cell(&m, rec_xz[2 * (rec_off + g)], rec_xz[2 * (rec_off + g) + 1]);
+        /* forward sweep: checkpoints 1 .. nseg-1 */
         for (long i = 0; i < N; i++) { s.p[i] = s.px[i] = s.pz[i] = s.vx[i] = s.vz[i] = 0.0; }
-        for (int n = 0; n < nt; n++) {
-            memcpy(Vx + (long)n * N, s.vx, sizeof(double) * N);
-            memcpy(Vz + (long)n * N, s.vz, sizeof(double) * N);
+        memset(ck, 0, sizeof(double) * 4 * N);
+        for (int n = 0; n < (nseg - 1) * K; n++) {
             orc_step(&m, &s, n, isrc, 1.0);
-            memcpy(Pn + (long)n * N, s.p, sizeof(double) * N);
+            if ((n + 1) % K == 0) {
+                double *c = ck + (size_t)4 * N * ((n + 1) / K);
+                memcpy(c, s.px, 8 * N); memcpy(c + N, s.pz, 8 * N);
+                memcpy(c + 2 * N, s.vx, 8 * N); memcpy(c + 3 * N, s.vz, 8 * N);
+            }
         }
         for (long i = 0; i < N; i++) { lpx[i] = lpz[i] = lvx[i] = lvz[i] = 0.0; }
-        for (int n = nt - 1; n >= 0; n--) {
-            const double *vxo = Vx + (long)n * N, *vzo = Vz + (long)n * N, *pn = Pn + (long)n * N;
-            /* record (cpu_record after step n) */
-            for (long i = 0; i < N; i++) lp[i] = 0.0;
-            if ((n + 1) % m.dt_factor == 0) {
-                const int k = (n + 1) / m.dt_factor - 1;
-                if (k < m.real_nt)
-                    for (int g = 0; g < ng; g++) lp[gc[g]] += adj[a_off + (long)g * m.real_nt + k];
+        for (int seg = nseg - 1; seg >= 0; seg--) {
+            const int n0 = seg * K, n1 = (n0 + K < nt) ? n0 + K : nt;
+            /* recompute the segment's states from its checkpoint */
+            const double *c = ck + (size_t)4 * N * seg;
+            memcpy(s.px, c, 8 * N); memcpy(s.pz, c + N, 8 * N);
+            memcpy(s.vx, c + 2 * N, 8 * N); memcpy(s.vz, c + 3 * N, 8 * N);
+            for (int n = n0; n < n1; n++) {
+                memcpy(Vx + (long)(n - n0) * N, s.vx, sizeof(double) * N);
+                memcpy(Vz + (long)(n - n0) * N, s.vz, sizeof(double) * N);
+                orc_step(&m, &s, n, isrc, 1.0);
+                memcpy(Pn + (long)(n - n0) * N, s.p, sizeof(double) * N);
             }
-            /* v-update: vx' = damp2b vx + buw2 Dx+ p ;  vz' = damp1b vz + buw1 Dz+ p */
-            for (int ix = 0; ix < nnx; ix++)
-                for (int iz = 0; iz < nnz; iz++) {
-                    const long id = iz + (long)ix * nnz;
-                    t1[id] = m.buw2[id] * lvx[id];
-                    t2[id] = m.buw1[id] * lvz[id];
-                    gb2[id] += lvx[id] * dplus_at(pn + id, ix, nnx, nnz) * m.dtdx * m.damp2b1[ix];
-                    gb1[id] += lvz[id] * dplus_at(pn + id, iz, nnz, 1) * m.dtdz * m.damp1b1[iz];
-                    nvx[id] = m.damp2b[ix] * lvx[id];
-                    nvz[id] = m.damp1b[iz] * lvz[id];
+            for (int n = n1 - 1; n >= n0; n--) {
+                const double *vxo = Vx + (long)(n - n0) * N, *vzo = Vz + (long)(n - n0) * N;
+                const double *pn = Pn + (long)(n - n0) * N;
+                /* record (cpu_record after step n) */
+                #pragma omp parallel for schedule(static)
+                for (long i = 0; i < N; i++) lp[i] = 0.0;
+                if ((n + 1) % m.dt_factor == 0) {
+                    const int k = (n + 1) / m.dt_factor - 1;
+                    if (k < m.real_nt)
+                        for (int g = 0; g < ng; g++) lp[gc[g]] += adj[a_off + (long)g * m.real_nt + k];
                 }
-            for (int iz = 0; iz < nnz; iz++) dplus_T_scatter(t1, lp, nnx, nnz, iz);
-            for (int ix = 0; ix < nnx; ix++) dplus_T_scatter(t2, lp, nnz, 1, (long)ix * nnz);
-            /* p = px + pz, then the source on px */
-            for (long i = 0; i < N; i++) { lpx[i] += lp[i]; lpz[i] += lp[i]; }
-            if (n < m.nts) gk[isrc] += lpx[isrc] * (m.fd_dt * m.wlt[n] / (m.dx * m.dz));
-            /* p-update: px' = damp2a px + kappaw2 Dx- vx ;  pz' = damp1a pz + kappaw1 Dz- vz */
-            for (int ix = 0; ix < nnx; ix++)
-                for (int iz = 0; iz < nnz; iz++) {
-                    const long id = iz + (long)ix * nnz;
-                    gk[id] += lpx[id] * dminus_at(vxo + id, ix, nnx, nnz) * m.dtdx * m.damp2a1[ix]
-                            + lpz[id] * dminus_at(vzo + id, iz, nnz, 1) * m.dtdz * m.damp1a1[iz];
-                    t1[id] = m.kappaw2[id] * lpx[id];
-                    t2[id] = m.kappaw1[id] * lpz[id];
-                    lpx[id] *= m.damp2a[ix];
-                    lpz[id] *= m.damp1a[iz];
-                }
-            for (int iz = 0; iz < nnz; iz++) dminus_T_scatter(t1, nvx, nnx, nnz, iz);
-            for (int ix = 0; ix < nnx; ix++) dminus_T_scatter(t2, nvz, nnz, 1, (long)ix * nnz);
-            memcpy(lvx, nvx, sizeof(double) * N);
-            memcpy(lvz, nvz, sizeof(double) * N);
+                /* v-update: vx' = damp2b vx + buw2 Dx+ p ;  vz' = damp1b vz + buw1 Dz+ p */
+                #pragma omp parallel for schedule(static)
+                for (int ix = 0; ix < nnx; ix++)
+                    for (int iz = 0; iz < nnz; iz++) {
+                        const long id = iz + (long)ix * nnz;
+                        t1[id] = m.buw2[id] * lvx[id];
+                        t2[id] = m.buw1[id] * lvz[id];
+                        gb2[id] += lvx[id] * dplus_at(pn + id, ix, nnx, nnz) * m.dtdx * m.damp2b1[ix];
+                        gb1[id] += lvz[id] * dplus_at(pn + id, iz, nnz, 1) * m.dtdz * m.damp1b1[iz];
+                        nvx[id] = m.damp2b[ix] * lvx[id];
+                        nvz[id] = m.damp1b[iz] * lvz[id];
+                    }
+                #pragma omp parallel for schedule(static)
+                for (int iz = 0; iz < nnz; iz++) dplus_T_scatter(t1, lp, nnx, nnz, iz);
+                #pragma omp parallel for schedule(static)
+                for (int ix = 0; ix < nnx; ix++) dplus_T_scatter(t2, lp, nnz, 1, (long)ix * nnz);
+                /* p = px + pz, then the source on px */
+                #pragma omp parallel for schedule(static)
+                for (long i = 0; i < N; i++) { lpx[i] += lp[i]; lpz[i] += lp[i]; }
+                if (n < m.nts) gk[isrc] += lpx[isrc] * (m.fd_dt * m.wlt[n] / (m.dx * m.dz));
+                /* p-update: px' = damp2a px + kappaw2 Dx- vx ;  pz' = damp1a pz + kappaw1 Dz- vz */
+                #pragma omp parallel for schedule(static)
+                for (int ix = 0; ix < nnx; ix++)
+                    for (int iz = 0; iz < nnz; iz++) {
+                        const long id = iz + (long)ix * nnz;
+                        gk[id] += lpx[id] * dminus_at(vxo + id, ix, nnx, nnz) * m.dtdx * m.damp2a1[ix]
+                                + lpz[id] * dminus_at(vzo + id, iz, nnz, 1) * m.dtdz * m.damp1a1[iz];
+                        t1[id] = m.kappaw2[id] * lpx[id];
+                        t2[id] = m.kappaw1[id] * lpz[id];
+                        lpx[id] *= m.damp2a[ix];
+                        lpz[id] *= m.damp1a[iz];
+                    }
+                #pragma omp parallel for schedule(static)
+                for (int iz = 0; iz < nnz; iz++) dminus_T_scatter(t1, nvx, nnx, nnz, iz);
+                #pragma omp parallel for schedule(static)
+                for (int ix = 0; ix < nnx; ix++) dminus_T_scatter(t2, nvz, nnz, 1, (long)ix * nnz);
+                memcpy(lvx, nvx, sizeof(double) * N);
+                memcpy(lvz, nvz, sizeof(double) * N);
+            }
         }
         free(gc);
         rec_off += ng;
@@ -720,9 +777,19 @@ int orc_gradient(const orc_params *P, const float *vp, const float *rho, int ns,
             g_rho[(long)mx * P->nz + mz] += gr;
         }
     free(vpe); free(rhoe); free(gb);
-    free(gk); free(gb1); free(gb2); free(Vx); free(Vz); free(Pn);
+    free(gk); free(gb1); free(gb2); free(Vx); free(Vz); free(Pn); free(ck);
     free(lpx); free(lpz); free(lvx); free(lvz); free(lp); free(t1); free(t2); free(nvx); free(nvz);
     orc_state_free(&s);
     orc_model_free(&m);
     return 0;
+}
+
+/* The same gradient with every forward state of the record kept (no
+ * recomputation). */
+int orc_gradient(const orc_params *P, const float *vp, const float *rho, int ns,
+                 const float *src_xz, const int *nrec, const float *rec_xz, const float *adj,
+                 double *g_vp, double *g_rho, double *gk_out, double *gb1_out, double *gb2_out)
+{
+    return orc_gradient_k(P, vp, rho, ns, src_xz, nrec, rec_xz, adj, 0, g_vp, g_rho, gk_out,
+                          gb1_out, gb2_out);
 }

@@ -7,8 +7,6 @@ Produces
   seisacomod2d_amd/libsam2d_host.so        include/sam2d_modelling.h: job card, geometry,
                                            model files, modelling_module drop-in
   seisacomod2d_amd/SeisAcoMod2D_MI355X.exe the job-card driver (reference CLI equivalent)
-  seisacomod2d_amd/stream_probe            memory-system ceiling of the kernels' access
-                                           pattern (bench.py --stream-probe)
 """
 import os
 import shutil
@@ -84,16 +82,13 @@ def build(force=True, jobs=8):
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-o", exe,
           os.path.join(HOST, "main.cpp"), "-L" + PKG, "-lsam2d_host", "-lsam2d",
           "-Wl,-rpath,$ORIGIN"])
-    probe = os.path.join(PKG, "stream_probe")
-    _run([HIPCC, "--offload-arch=" + ARCH, "-O3", "-std=c++17"] + DEVICE_FLAGS +
-         ["-o", probe + ".tmp", os.path.join(CSRC, "stream_probe.hip")])
-    os.replace(probe + ".tmp", probe)
     return lib
 
 
 def build_oracle():
     """Compile the CPU checker (oracle/) and, where the reference tree is
-    present, the reference executable into oracle/_ref/."""
+    present, the reference executable and the reference front end linked
+    against libsam2d_host.so into oracle/_ref/."""
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-C", odir, "liboracle.so"])
     if os.path.isdir("/root/reference/src/src_cpu") and shutil.which("make"):

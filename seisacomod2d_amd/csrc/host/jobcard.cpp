@@ -19,7 +19,7 @@ mod_t *mod_sp = &g_mod;
 wave_t *wave_sp = &g_wave;
 geo2d_t *geo2d_sp = &g_geo;
 job_t *job_sp = &g_job;
-sam2d_run_options sam2d_opts = {32, 0, 1};
+sam2d_run_options sam2d_opts = {32, 0, 1, 0, 0};
 
 // json_parser.cpp:13-37: the card's file name without directory and
 // extension, followed by "_".

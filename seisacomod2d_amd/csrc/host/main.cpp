@@ -1,9 +1,10 @@
 // main.cpp -- SeisAcoMod2D_MI355X.exe <jobcard> [--npml N] [--per-shot-files]
-//             [--no-restart-file]
+//             [--no-restart-file] [--reset-quirk] [--su] [--check]
 // The reference's master-rank flow (main.cpp:52-90, modelling_master.cpp:10-91)
 // in one process on one GPU: job card, geometry, models, stability check,
-// modelling_module.  Exits with the reference's abort code negated (e.g. 5
-// for a CFL violation, the reference's Abort(-5)).
+// modelling_module.  Ends with the exit status the reference's
+// MPI::COMM_WORLD.Abort(code) gives under MPICH, code & 0xff (251 for a CFL
+// violation, the reference's Abort(-5)); 0 on success.
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -17,10 +18,14 @@ static int run(int argc, char **argv)
         std::cerr << "\n Error!!! Insufficient number of argument\n Please provide input parameter file\n";
         return -1;
     }
+    bool check_only = false;
     for (int a = 2; a < argc; a++) {
         if (!std::strcmp(argv[a], "--npml") && a + 1 < argc) sam2d_opts.npml = std::atoi(argv[++a]);
         else if (!std::strcmp(argv[a], "--per-shot-files")) sam2d_opts.per_shot_files = 1;
         else if (!std::strcmp(argv[a], "--no-restart-file")) sam2d_opts.write_restart = 0;
+        else if (!std::strcmp(argv[a], "--reset-quirk")) sam2d_opts.reset_quirk = 1;
+        else if (!std::strcmp(argv[a], "--su")) sam2d_opts.su_output = 1;
+        else if (!std::strcmp(argv[a], "--check")) check_only = true;
         else { std::cerr << "unknown option " << argv[a] << "\n"; return -6; }
     }
     const auto t0 = std::chrono::steady_clock::now();
@@ -44,7 +49,11 @@ static int run(int argc, char **argv)
     minmax2d(mod_sp->vel2d, mod_sp->nx, mod_sp->nz, &mod_sp->vmin, &mod_sp->vmax);
     std::cout << "\n Min. Velocity : " << mod_sp->vmin << "\t Max. Velocity : " << mod_sp->vmax;
     if ((rc = check_stability())) return rc;
-    rc = modelling_module(geo2d_sp->nsrc, geo2d_sp);
+    if (check_only) {
+        std::cout << "\n --check: inputs accepted, modelling skipped\n";
+        return 0;
+    }
+    rc = sam2d_modelling_module(geo2d_sp->nsrc, geo2d_sp);
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::cout << "\n Total time of execution : " << s / 3600 << "  .Hrs (" << s << " s)\n";
     return rc;
@@ -54,5 +63,5 @@ int main(int argc, char **argv)
 {
     const int rc = run(argc, argv);
     if (rc) std::cerr << "\n aborting with code " << rc << "\n";
-    return rc < 0 ? -rc : rc;
+    return rc & 0xff;
 }

@@ -435,6 +435,7 @@ extern "C" int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshot
 
     std::vector<ShotPlan> plans;
     SAM2D_TRY(make_plans(ctx, shots, nshots, windows, plans));
+    ctx->quirk_valid = false;
     std::vector<const float *> adj_ptr(nshots);
     std::vector<float *> seis_ptr(nshots);
     size_t off = 0;

@@ -477,24 +477,27 @@ int forward_plans(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int batch, float
 
 // The reference's carry-over between shots (cpumodelling.cpp:343-348): the
 // next shot starts from the previous final state with only the first nnz
-// cells of p, px, pz, vx, vz cleared.  Per-shot loop only, no graph.
+// cells of p, px, pz, vx, vz cleared.  The previous shot is the last one this
+// context modelled in this mode, in this call or an earlier one (the
+// reference's modelling_module runs its shots one after another on the same
+// arrays); any other modelling call ends the chain.  Per-shot loop only, no
+// graph.
 static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, float *seis)
 {
     const int real_nt = ctx->ta.real_nt;
     size_t off = 0;
     const int variant = pick_variant(ctx);
-    int fin = 0;    // slot holding the previous shot's final state
     for (size_t s = 0; s < plans.size(); s++) {
         std::vector<ShotPlan> one(1, plans[s]);
         // Keep the previous state: stage into slot 0 without its clearing.
         ShotBuffers *b0 = ctx->bufs.empty() || !ctx->bufs[0] ? nullptr : ctx->bufs[0].get();
         const size_t N = (size_t)one[0].dom->host.nnx * one[0].dom->host.nnz;
         std::vector<float> keep;
-        if (s > 0 && b0) {
+        if (ctx->quirk_valid && ctx->quirk_cells == N && b0) {
             // Bring the final state into set 0, then clear the first nnz cells.
             keep.resize(4 * N);
-            HIP_TRY(hipMemcpyAsync(keep.data(), b0->slot(fin), 4 * N * 4, hipMemcpyDeviceToHost,
-                                   ctx->stream));
+            HIP_TRY(hipMemcpyAsync(keep.data(), b0->slot(ctx->quirk_slot), 4 * N * 4,
+                                   hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             const int nnz = one[0].dom->host.nnz;
             if (ctx->state_aos)      // cells 0 .. nnz-1, four fields each
@@ -503,13 +506,15 @@ static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, flo
                 for (int f = 0; f < 4; f++) std::fill(&keep[f * N], &keep[f * N] + nnz, 0.0f);
         }
         int ntiles = 0;
+        ctx->quirk_valid = false;
         SAM2D_TRY(stage_batch(ctx, one, variant, &ntiles));
-        if (s > 0 && !keep.empty()) {
+        if (!keep.empty()) {
             ShotBuffers &b = *ctx->bufs[0];
             SAM2D_TRY(upload(b.slot(0), keep.data(), 4 * N * 4, ctx->stream));
         }
         ctx->stats.cell_updates += (double)N * ctx->ta.fd_nt;
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+        int fin = 0;
         SAM2D_TRY(enqueue_steps(ctx, 1, ntiles, variant, 0, ctx->ta.fd_nt, one[0].ng, 0, &fin));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(hipEventSynchronize(ctx->ev1));
@@ -521,6 +526,9 @@ static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, flo
         const size_t n = (size_t)one[0].ng * real_nt;
         if (n) HIP_TRY(hipMemcpy(seis + off, one[0].seis_dev, n * 4, hipMemcpyDeviceToHost));
         off += n;
+        ctx->quirk_valid = true;
+        ctx->quirk_slot = fin;
+        ctx->quirk_cells = N;
     }
     return SAM2D_OK;
 }
@@ -667,6 +675,7 @@ static int forward_common(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots, c
     if (ctx->reset_quirk && !batched && !dev_out && !windows) {
         SAM2D_TRY(forward_reset_quirk(ctx, plans, seis));
     } else {
+        ctx->quirk_valid = false;
         int batch = 1;
         if (batched) {
             batch = ctx->max_batch > 0 ? ctx->max_batch : nshots;
@@ -711,7 +720,10 @@ int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value)
     else if (k == "variant") {
         if (value >= num_variants()) return fail(SAM2D_ERR_PARAM, "variant %d out of range", value);
         ctx->variant = value < 0 ? -1 : value;
-    } else if (k == "reset_quirk") ctx->reset_quirk = value != 0;
+    } else if (k == "reset_quirk") {
+        ctx->reset_quirk = value != 0;
+        ctx->quirk_valid = false;            // the next shot starts fresh
+    }
     else if (k == "max_batch") ctx->max_batch = value < 0 ? 0 : value;
     else if (k == "grid_cap") ctx->grid_cap = value < 0 ? 0 : value;
     else if (k == "hbm_budget_gb") {
@@ -777,6 +789,7 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
     }
     std::vector<ShotPlan> plans;
     SAM2D_TRY(make_plans(ctx, shots.data(), nshots, windows, plans));
+    ctx->quirk_valid = false;
     int batch = ctx->max_batch > 0 ? std::min(ctx->max_batch, nshots) : nshots;
     plans.resize(batch);
     double best_ms = 1e300;
@@ -788,7 +801,12 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
         int waves, chunk;
         variant_shape(v, &waves, &chunk);
         int ntiles = 0;
-        SAM2D_TRY(stage_batch(ctx, plans, v, &ntiles));
+        const int rc = stage_batch(ctx, plans, v, &ntiles);
+        if (rc == SAM2D_ERR_PARAM && variant_steps(v) > 1) {
+            clear_error();             // grid beyond the multi-step limit: not a candidate
+            continue;
+        }
+        SAM2D_TRY(rc);
         SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, v, 0, 2, 0, 0, nullptr));   // warm
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
         SAM2D_TRY(enqueue_steps(ctx, batch, ntiles, v, 2, 2 + steps, 0, 0, nullptr));
@@ -832,7 +850,7 @@ int sam2d_autotune(sam2d_ctx *ctx, const int *windows, int nshots, int steps, in
     };
     std::vector<int> multi;
     for (int v = 0; v < num_variants(); v++)
-        if (variant_steps(v) > 1) multi.push_back(v);
+        if (variant_steps(v) > 1 && ctx->tune_ms[v] > 0.0) multi.push_back(v);
     std::sort(multi.begin(), multi.end(),
               [&](int a, int b) { return ctx->tune_ms[a] < ctx->tune_ms[b]; });
     if (variant_steps(best) > 1 && !multi.empty()) {

@@ -106,6 +106,11 @@ struct sam2d_ctx {
     std::map<sam2d::GraphKey, hipGraphExec_t> graphs;
     bool use_graph = true;
     bool reset_quirk = false;
+    // reset_quirk carry-over: the final state of the last shot modelled in
+    // that mode (slot quirk_slot of bufs[0], a domain of quirk_cells cells)
+    bool quirk_valid = false;
+    int quirk_slot = 0;
+    size_t quirk_cells = 0;
     int variant = -1, tuned_variant = -1, default_variant = 0;
     int max_batch = 0;
     int grid_cap = 0;

@@ -16,23 +16,43 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
 
 
-def golden_cases():
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+def _kind(path):
+    z = np.load(path)
+    return str(z["kind"]) if "kind" in z else "single"
+
+
+def golden_cases(kind="single"):
+    """Golden cases of one kind: "single" (one short shot, stored whole),
+    "long" (full-record configurations, stored decimated) or "two_shot"."""
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                  if _kind(p) == kind)
 
 
 def load_golden(name):
     """Inputs and reference output of one golden case (see tests/golden/make_goldens.py)."""
+    import hashlib
+    from seisacomod2d_amd import synthetic
     z = np.load(os.path.join(GOLDEN, name + ".npz"))
     nx, nz, dx, dz, T, fd_dt, real_dt, fm = z["params"]
     nx, nz = int(nx), int(nz)
+    model = str(z["model"]) if "model" in z else "two_layer"
+    extra = {}
     if "vp" in z:
         vp, rho = z["vp"], z["rho"]
-    else:  # analytic two-layer model of config 0
-        vp = np.full((nx, nz), 1500, np.float32); vp[:, nz // 2:] = 3000
-        rho = np.full((nx, nz), 1000, np.float32); rho[:, nz // 2:] = 2000
+    elif model == "salt_window":
+        # the window of the seeded salt-type model that the reference ran on
+        wx0, w = (int(v) for v in z["window"])
+        vp_all, rho_all = synthetic.salt_model()
+        vp = np.ascontiguousarray(vp_all[wx0:wx0 + w])
+        rho = np.ascontiguousarray(rho_all[wx0:wx0 + w])
+        assert hashlib.sha256(vp.tobytes() + rho.tobytes()).hexdigest() == str(z["model_sha256"]), \
+            "regenerated salt model differs from the one the reference ran on"
+        extra = dict(window=(wx0, w), vp_full=vp_all, rho_full=rho_all)
+    else:  # analytic two-layer model, interface at nz // 2
+        vp, rho = synthetic.two_layer(nx, nz)
     return dict(nx=nx, nz=nz, dx=float(dx), dz=float(dz), rec_len=float(T), fd_dt=float(fd_dt),
                 real_dt=float(real_dt), fm=float(fm), vp=vp, rho=rho, src_xz=z["src_xz"],
-                rec_xz=z["rec_xz"], seis=z["seis"], trace_idx=z["trace_idx"])
+                rec_xz=z["rec_xz"], seis=z["seis"], trace_idx=z["trace_idx"], **extra)
 
 
 # Seismogram comparison used by every parity test.
@@ -52,6 +72,12 @@ def trace_errors(test, ref):
     scale = np.maximum(nrm, FLOOR * mx)
     scale = np.where(scale > 0, scale, 1.0)
     return err / scale
+
+
+def floored_traces(ref):
+    """Number of traces held to the gather-relative floor instead of their own norm."""
+    nrm = np.linalg.norm(np.asarray(ref, np.float64), axis=-1)
+    return int(np.sum(nrm < FLOOR * nrm.max())) if nrm.size else 0
 
 
 def assert_seis_close(test, ref, tol, what=""):

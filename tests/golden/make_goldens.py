@@ -1,6 +1,6 @@
 """Write golden seismograms with the compiled reference executable.
 
-    make -C oracle ref && python tests/golden/make_goldens.py
+    make -C oracle ref && python tests/golden/make_goldens.py [case ...]
 
 Runs oracle/_ref/SeisAcoMod2D_MULTI_CORE_CPU_V1.0.exe (src/src_cpu of the
 reference, built from its sources by oracle/Makefile.ref) on small job cards
@@ -16,9 +16,15 @@ output as tests/golden/<case>.npz:
            <job>_seismogram.bin), possibly decimated in traces: trace_idx
   geometry_text  the shot-file line or explicit-geometry lines of the card
 
+  kind     "long" for the full-record cases (stored decimated, with the
+           SHA-256 of the whole output), "two_shot" for the carry-over case
+  model    how the tests rebuild a model that is not stored ("two_layer",
+           "salt_window" with window = (first column, width))
+
 The reference fixes npml = 32 (cpumodelling.cpp:15), so every golden is at
-npml = 32.  Runs are single-shot: multi-shot runs of the reference overwrite
-one output file and carry state between shots (cpumodelling.cpp:343-374).
+npml = 32.  Runs are single-shot except two_shot_carryover: a multi-shot run
+of the reference rewrites one output file per shot and carries state between
+shots (cpumodelling.cpp:343-374), which that case records on purpose.
 """
 import hashlib
 import os
@@ -32,6 +38,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 
 sys.path.insert(0, ROOT)
 from oracle import refrun  # noqa: E402
+from seisacomod2d_amd import synthetic  # noqa: E402
 
 
 def run_reference(nx, nz, dx, dz, T, fd_dt, real_dt, fm, vp, rho, shot_line=None, geom_lines=None,
@@ -93,11 +100,10 @@ def save(name, **kw):
     print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
 
 
-def main(with_c0=True):
-    if not refrun.available():
-        sys.exit(f"{refrun.EXE} missing: make -C oracle ref")
 
-    # 1. two-layer, split spread from a shot file, dt_factor 1
+
+def case_tiny_twolayer():
+    # two-layer, split spread from a shot file, dt_factor 1
     nx, nz, h = 100, 80, 10.0
     vp, rho = two_layer(nx, nz, 40)
     src, rec, seis, _ = run_reference(nx, nz, h, h, 0.3, 0.001, 0.001, 15, vp, rho,
@@ -105,7 +111,9 @@ def main(with_c0=True):
     save("tiny_twolayer", params=params_arr(nx, nz, h, h, 0.3, 0.001, 0.001, 15), vp=vp, rho=rho,
          src_xz=src, rec_xz=rec, seis=seis, trace_idx=np.arange(seis.shape[0]))
 
-    # 2. dx != dz, DATA DT = 4 x FD DT, end-on-right spread near the left edge
+
+def case_rect_dtfactor():
+    # dx != dz, DATA DT = 4 x FD DT, end-on-right spread near the left edge
     nx, nz = 120, 90
     vp, rho = two_layer(nx, nz, 50)
     src, rec, seis, _ = run_reference(nx, nz, 15.0, 10.0, 0.4, 0.001, 0.004, 10, vp, rho,
@@ -113,8 +121,10 @@ def main(with_c0=True):
     save("rect_dtfactor", params=params_arr(nx, nz, 15.0, 10.0, 0.4, 0.001, 0.004, 10), vp=vp,
          rho=rho, src_xz=src, rec_xz=rec, seis=seis, trace_idx=np.arange(seis.shape[0]))
 
-    # 3. heterogeneous vp and rho, explicit geometry with receivers at several
-    #    depths, at the model edges and on the source cell
+
+def case_hetero_explicit():
+    # heterogeneous vp and rho, explicit geometry with receivers at several
+    # depths, at the model edges and on the source cell
     nx, nz, h = 150, 100, 10.0
     vp, rho = smooth_random(nx, nz, 7)
     sx, sz = 740.0, 300.0
@@ -126,7 +136,9 @@ def main(with_c0=True):
     save("hetero_explicit", params=params_arr(nx, nz, h, h, 0.45, 0.0008, 0.0016, 12), vp=vp,
          rho=rho, src_xz=src, rec_xz=rec, seis=seis, trace_idx=np.arange(seis.shape[0]))
 
-    # 4. salt-type body at 7.62 m, dt 0.5 ms (shot-file geometry, float spacing)
+
+def case_salt_small():
+    # salt-type body at 7.62 m, dt 0.5 ms (shot-file geometry, float spacing)
     nx, nz, h = 200, 120, 7.62
     vp, rho = salt(nx, nz, h, 11)
     src, rec, seis, _ = run_reference(nx, nz, h, h, 0.35, 0.0005, 0.0005, 15, vp, rho,
@@ -134,19 +146,107 @@ def main(with_c0=True):
     save("salt_small", params=params_arr(nx, nz, h, h, 0.35, 0.0005, 0.0005, 15), vp=vp, rho=rho,
          src_xz=src, rec_xz=rec, seis=seis, trace_idx=np.arange(seis.shape[0]))
 
-    if with_c0:
-        # 5. config 0 at the reference's npml = 32: 1000 x 1000, 2 s, 1000
-        #    receivers every cell at z = 20 m.  Stored decimated (every 20th
-        #    trace) with the SHA-256 of the full output.
-        nx = nz = 1000
-        vp, rho = two_layer(nx, nz, 500)
-        src, rec, seis, _ = run_reference(nx, nz, 10.0, 10.0, 2.0, 0.001, 0.001, 15, vp, rho,
-                                          shot_line="5000.000000,20.000000,1000,0,10.000000,1")
-        idx = np.arange(0, seis.shape[0], 20)
-        save("c0_twolayer_npml32", params=params_arr(nx, nz, 10.0, 10.0, 2.0, 0.001, 0.001, 15),
-             src_xz=src, rec_xz=rec, seis=seis[idx], trace_idx=idx,
-             sha256=np.array(hashlib.sha256(seis.tobytes()).hexdigest()))
+
+def case_c0_twolayer_npml32():
+    # config 0 at the reference's npml = 32: 1000 x 1000, 2 s, 1000 receivers
+    # every cell at z = 20 m.  Stored decimated (every 20th trace) with the
+    # SHA-256 of the full output.
+    nx = nz = 1000
+    vp, rho = two_layer(nx, nz, 500)
+    src, rec, seis, _ = run_reference(nx, nz, 10.0, 10.0, 2.0, 0.001, 0.001, 15, vp, rho,
+                                      shot_line="5000.000000,20.000000,1000,0,10.000000,1")
+    idx = np.arange(0, seis.shape[0], 20)
+    save("c0_twolayer_npml32", params=params_arr(nx, nz, 10.0, 10.0, 2.0, 0.001, 0.001, 15),
+         src_xz=src, rec_xz=rec, seis=seis[idx], trace_idx=idx,
+         sha256=np.array(hashlib.sha256(seis.tobytes()).hexdigest()))
+
+
+def case_two_shot_carryover():
+    # Two shots on one explicit-geometry card.  The reference writes every
+    # shot to the same <job>_seismogram.bin (cpumodelling.cpp:367-374), so the
+    # file it leaves holds shot 2, and shot 2 starts from shot 1's final
+    # wavefield with only the first nnz cells cleared (cpumodelling.cpp:343-348).
+    # The record is short enough that shot 1's wavefield is still strong in
+    # the interior when shot 2 starts.
+    nx, nz, h = 120, 80, 10.0
+    vp, rho = two_layer(nx, nz, 40)
+    gx = np.arange(0, nx) * h
+    shots = [(400.0, 20.0), (800.0, 30.0)]
+    lines = [f"{sx:f},{sz:f},{x:f},{20.0:f}" for sx, sz in shots for x in gx]
+    _, _, seis, _ = run_reference(nx, nz, h, h, 0.25, 0.001, 0.001, 15, vp, rho, geom_lines=lines)
+    rec = np.stack([gx, np.full(nx, 20.0)], 1).astype(np.float32)
+    save("two_shot_carryover", params=params_arr(nx, nz, h, h, 0.25, 0.001, 0.001, 15), vp=vp,
+         rho=rho, src_xz=np.array(shots, np.float32), rec_xz=rec, seis=seis,
+         trace_idx=np.arange(seis.shape[0]), kind=np.array("two_shot"))
+
+
+def case_c1_full_npml32():
+    # The 5000 x 5000 two-layer configuration over its whole 4 s record
+    # (4000 steps) at the reference's sponge of 32 cells, one shot (the
+    # 33rd of the 64 evenly spaced positions), 5000 receivers every cell at
+    # z = 20 m.  Stored: every 100th trace at full length, and the SHA-256 of
+    # the whole output file.
+    nx = nz = 5000
+    vp, rho = synthetic.two_layer(nx, nz)
+    sx, sz, _ = synthetic.surface_spread_shots(nx, 10.0, 64)[32]
+    src, rec, seis, _ = run_reference(nx, nz, 10.0, 10.0, 4.0, 0.001, 0.001, 15, vp, rho,
+                                      shot_line=f"{sx:f},{sz:f},{nx},0,10.000000,1")
+    idx = np.arange(0, seis.shape[0], 100)
+    save("c1_full_npml32", params=params_arr(nx, nz, 10.0, 10.0, 4.0, 0.001, 0.001, 15),
+         src_xz=src, rec_xz=rec, seis=seis[idx], trace_idx=idx,
+         sha256=np.array(hashlib.sha256(seis.tobytes()).hexdigest()), kind=np.array("long"),
+         model=np.array("two_layer"))
+
+
+def case_c2_window_npml32():
+    # One aperture window of the salt-type configuration (7.62 m, 0.5 ms,
+    # 8000 steps, 15 Hz) as the reference models it: the window cropped out
+    # of the 3201 x 1201 model into its own model file, the reference's
+    # sponge of 32 cells around it, an explicit geometry file with the
+    # source and a receiver at every cell of the window (coordinates
+    # relative to the window, on grid nodes as the reference's geometry
+    # check requires, read_receiver.cpp:96-121).  The model is regenerated
+    # from its seed by the tests (synthetic.salt_model) and checked against
+    # the stored hash.
+    vp_all, rho_all = synthetic.salt_model()
+    shots, wins = synthetic.salt_shots(64)
+    k = 20
+    wx0, w = wins[k]
+    h = 7.62
+    vp = np.ascontiguousarray(vp_all[wx0:wx0 + w])
+    rho = np.ascontiguousarray(rho_all[wx0:wx0 + w])
+    sx, sz, rec = shots[k]
+    isx = int(sx / h) - wx0
+    lines = [f"{isx * h:f},{2 * h:f},{i * h:f},{2 * h:f}" for i in range(w)]
+    src, rec_ref, seis, _ = run_reference(w, 1201, h, h, 4.0, 0.0005, 0.0005, 15, vp, rho,
+                                          geom_lines=lines)
+    idx = np.arange(0, seis.shape[0], 60)
+    save("c2_window_npml32", params=params_arr(w, 1201, h, h, 4.0, 0.0005, 0.0005, 15),
+         src_xz=src, rec_xz=rec_ref, seis=seis[idx], trace_idx=idx,
+         sha256=np.array(hashlib.sha256(seis.tobytes()).hexdigest()), kind=np.array("long"),
+         model=np.array("salt_window"), window=np.array([wx0, w]), shot_index=np.array(k),
+         model_sha256=np.array(hashlib.sha256(vp.tobytes() + rho.tobytes()).hexdigest()))
+
+
+CASES = {
+    "tiny_twolayer": case_tiny_twolayer,
+    "rect_dtfactor": case_rect_dtfactor,
+    "hetero_explicit": case_hetero_explicit,
+    "salt_small": case_salt_small,
+    "c0_twolayer_npml32": case_c0_twolayer_npml32,
+    "two_shot_carryover": case_two_shot_carryover,
+    "c1_full_npml32": case_c1_full_npml32,
+    "c2_window_npml32": case_c2_window_npml32,
+}
+
+
+def main(names):
+    if not refrun.available():
+        sys.exit(f"{refrun.EXE} missing: make -C oracle ref")
+    for n in names or CASES:
+        print(f"# {n}", flush=True)
+        CASES[n]()
 
 
 if __name__ == "__main__":
-    main(with_c0="--no-c0" not in sys.argv)
+    main(sys.argv[1:])

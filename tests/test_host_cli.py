@@ -1,10 +1,13 @@
 """CPU tests of the reference-shaped host layer (libsam2d_host.so,
-include/sam2d_modelling.h): job card, shot-file and explicit geometry, model
-files, stability check -- against the geometry the compiled reference wrote
-for the golden cases (its test_geometry.txt dump) and its abort codes."""
+include/sam2d_modelling.h) and the job-card driver: job card, shot-file and
+explicit geometry, model files, stability check -- against what the compiled
+reference itself wrote and returned: its test_geometry.txt dumps (golden cases
+and tests/golden/reference_cards.json) and its exit statuses."""
 import ctypes
+import json
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -63,12 +66,16 @@ def _write_card(d, g, flag, text, jobtype="New", extra=""):
 def test_host_library_exports_the_declared_functions(host):
     txt = open(os.path.join(ROOT, "include", "sam2d_modelling.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    names = re.findall(r"^\s*(?:int|void)\s+(\w+)\s*\(", txt, flags=re.M)
-    assert "modelling_module" in names and "read_job_card" in names
+    c_part = txt.split("#ifdef __cplusplus\n}")[0]
+    names = re.findall(r"^\s*(?:int|void)\s+(\w+)\s*\(", c_part, flags=re.M)
+    assert "sam2d_modelling_module" in names and "read_job_card" in names
     for n in names:
         assert hasattr(host, n), n
     for n in ("mod_sp", "wave_sp", "geo2d_sp", "job_sp", "sam2d_opts"):
         ctypes.c_void_p.in_dll(host, n)
+    # the reference's seam, C++ linkage: void modelling_module(int, Geo2d *)
+    assert "void modelling_module(int my_nsrc, geo2d_t *mygeo2d_sp);" in txt
+    assert hasattr(host, "_Z16modelling_moduleiP5Geo2d")
 
 
 @pytest.mark.parametrize("case", golden_cases())
@@ -88,86 +95,45 @@ def test_geometry_matches_reference_dump(host, tmp_path, case):
     np.testing.assert_array_equal(geo[1:, ::-1].astype(np.float32), g["rec_xz"])
 
 
-def test_float_accumulated_spread_reproduced(host, tmp_path):
-    # 401 receivers every 15.24 m (the reference's sigsbee card): the float
-    # accumulation ends one slot early and the last receiver stays at (0, 0).
-    g = load_golden("salt_small")
-    g = dict(g, nx=3201, nz=1201, dx=7.62, dz=7.62, vp=np.full((3201, 1201), 2000, np.float32),
-             rho=np.full((3201, 1201), 2000, np.float32))
-    card = _write_card(str(tmp_path), g, 1, "9997.439453,7.620000,401,0,15.240000,1")
-    assert host.read_job_card(card.encode()) == 0
-    assert host.read_source2d(os.path.join(str(tmp_path), "shots.txt").encode()) == 0
-    dump = os.path.join(str(tmp_path), "geometry.txt")
-    host.write_geometry_dump(dump.encode())
-    geo = np.loadtxt(dump, ndmin=2)
-    assert geo.shape[0] == 402
-    assert geo[-1, 0] == 0 and geo[-1, 1] == 0         # the unwritten slot
-    assert geo[-2, 1] > 0
+# ---------------------------------------------------------------------------
+# Cards the compiled reference was run on (tests/golden/make_reference_cards.py):
+# the geometry it generated and the code it aborted with.  The job-card driver
+# runs the same flow (--check stops before the modelling, which needs a GPU)
+# and must end with the reference's exit status and write the same
+# test_geometry.txt.
+CARDS = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_cards.json")))["cases"]
+EXE = os.path.join(ROOT, "seisacomod2d_amd", "SeisAcoMod2D_MI355X.exe")
 
 
-@pytest.mark.parametrize("line,code", [
-    ("500.000000,20.000000,100,0,10.000000,4", -12),   # bad geometry type
-    ("500.000000,20.000000,0,0,10.000000,1", -18),     # nrec <= 0
-    ("505.000000,20.000000,10,0,10.000000,1", -13),    # source off-grid in x
-    ("500.000000,25.000000,10,0,10.000000,1", -14),    # source off-grid in z
-    ("500.000000,20.000000,10,0,15.000000,1", -15),    # spacing off-grid
-    ("500.000000,20.000000,10,0,10.000000", -25),      # field count
-    ("500.000000,20.000000,10,0,10.000000,1,7", -25),
-    ("500.000000,,10,0,10.000000,1", -19),             # missing field
-    ("10.000000,20.000000,50,0,10.000000,2", -3),      # end-on-left spread leaves the model
-])
-def test_shot_file_abort_codes(host, tmp_path, line, code):
-    g = load_golden("tiny_twolayer")
-    card = _write_card(str(tmp_path), g, 1, line)
-    assert host.read_job_card(card.encode()) == 0
-    assert host.read_source2d(os.path.join(str(tmp_path), "shots.txt").encode()) == code
-
-
-@pytest.mark.parametrize("lines,code", [
-    (["500,20,100,20", "400,20,100,20"], -22),          # not sorted by sx
-    (["500,20,100,20", "", "600,20,10,20"], -21),       # blank line
-    (["500,20,100,20,5"], -26),                         # field count
-    (["500,20,100,20", "500,20,2000,20"], -16),         # receiver outside in x
-    (["500,20,100,20", "500,20,105,20"], -13),          # receiver off-grid
-])
-def test_explicit_geometry_abort_codes(host, tmp_path, lines, code):
-    g = load_golden("tiny_twolayer")
-    card = _write_card(str(tmp_path), g, 0, "\n".join(lines))
-    assert host.read_job_card(card.encode()) == 0
-    assert host.read_recv2d(os.path.join(str(tmp_path), "geom.txt").encode()) == code
-
-
-def test_job_card_rules(host, tmp_path):
-    g = load_golden("tiny_twolayer")
+@pytest.mark.parametrize("case", CARDS, ids=[c["name"] for c in CARDS])
+def test_cli_matches_reference_on_cards(tmp_path, case):
+    from golden.make_reference_cards import write_inputs
     d = str(tmp_path)
-    card = _write_card(d, g, 1, "500.000000,20.000000,100,0,10.000000,1")
-    assert host.read_job_card(card.encode()) == 0
-    # first occurrence of a key wins; comment lines are ignored
-    txt = open(card).read()
-    open(card, "w").write(txt.replace('"NX" : "100"', '"NX" : "100"\n"NX" : "7"\n"NZ comment" : "1"'))
-    assert host.read_job_card(card.encode()) == 0
-    # missing key -> -6; DATA DT not a multiple of FD DT -> -6; bad job type -> -6
-    open(card, "w").write(txt.replace('"Frequency"', '"Freq"'))
-    assert host.read_job_card(card.encode()) == -6
-    open(card, "w").write(txt.replace('"DATA DT" : "0.001"', '"DATA DT" : "0.0015"'))
-    assert host.read_job_card(card.encode()) == -6
-    open(card, "w").write(txt.replace('"New"', '"new"'))
-    assert host.read_job_card(card.encode()) == -6
-    assert host.read_job_card(os.path.join(d, "nope.json").encode()) == -2
+    write_inputs(d, case["grid"], case["model"], case["flag"], case["text"], case["card_edit"])
+    # the job status is checked inside modelling_module, before the GPU is
+    # touched; every other abort happens before modelling
+    args = [] if case["name"].startswith("abort_restart") else ["--check"]
+    r = subprocess.run([EXE, "card.json"] + args, cwd=d, capture_output=True, text=True, timeout=120)
+    if case["exit_status"] < 0:
+        # The reference crashed (explicit_two_shots_uneven: master_workload
+        # copies nrec[0] receivers for every shot, workload.cpp:144); the
+        # driver reads every shot's own receivers and runs.
+        assert r.returncode == 0, r.stderr[-1000:]
+    else:
+        assert r.returncode == case["exit_status"], (r.returncode, case, r.stderr[-1000:])
+        if case["abort_code"] is not None:
+            assert r.returncode == case["abort_code"] & 0xFF
+    dump = os.path.join(d, "test_geometry.txt")
+    if case["geometry"] is None:
+        assert not os.path.exists(dump)
+    else:
+        assert open(dump).read() == case["geometry"]
 
 
-def test_model_reader_size_rule(host, tmp_path):
-    nx, nz = 5, 4
-    arr = (ctypes.POINTER(ctypes.c_float) * nx)()
-    rows = [(ctypes.c_float * nz)() for _ in range(nx)]
-    for i in range(nx):
-        arr[i] = ctypes.cast(rows[i], ctypes.POINTER(ctypes.c_float))
-    f = os.path.join(str(tmp_path), "m.bin")
-    data = np.arange(nx * nz, dtype=np.float32)
-    data.tofile(f)
-    host.read2d.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
-    assert host.read2d(f.encode(), arr, nx, nz) == 0
-    assert np.array_equal(np.array([list(r) for r in rows], np.float32).ravel(), data)   # x-major
-    data[:-1].tofile(f)
-    assert host.read2d(f.encode(), arr, nx, nz) == -2
-    assert host.read2d(os.path.join(str(tmp_path), "missing.bin").encode(), arr, nx, nz) == -2
+def test_reference_cards_cover_every_abort_site():
+    # Every abort code the reference's front end can raise on its inputs is
+    # among the recorded cards (json_parser.cpp, create_geometry.cpp,
+    # read_receiver.cpp, model.cpp, check_stability.cpp, cpumodelling.cpp).
+    codes = {c["abort_code"] for c in CARDS if c["abort_code"] is not None}
+    assert codes >= {-2, -3, -4, -5, -6, -10, -11, -12, -13, -14, -15, -16, -17, -18, -19, -20,
+                     -21, -22, -24, -25, -26}

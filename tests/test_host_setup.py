@@ -12,24 +12,28 @@ import seisacomod2d_amd as sam
 from conftest import ROOT, load_golden
 
 
-def declared_symbols():
-    syms = set()
-    inc = os.path.join(ROOT, "include")
-    for f in os.listdir(inc):
-        if f.endswith(".h"):
-            txt = open(os.path.join(inc, f)).read()
-            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-            syms |= set(re.findall(r"\b(sam2d_\w+)\s*\(", txt))
-            syms |= set(re.findall(r"\b(modelling_module|SeisAcoMod2D_\w+)\s*\(", txt))
-    return sorted(syms)
+def declared_symbols(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = txt.split("#ifdef __cplusplus\n}")[0]          # the C part
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\**\s*(\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_library_loads_and_exports_every_declared_symbol():
+    # include/sam2d.h is libsam2d.so's ABI, include/sam2d_modelling.h
+    # libsam2d_host.so's; each library exports every function its header
+    # declares (loading them needs no GPU).
     L = sam.lib()
-    syms = declared_symbols()
-    assert "sam2d_forward" in syms and "sam2d_gradient" in syms
-    missing = [s for s in syms if s != "modelling_module" and not hasattr(L, s)]
+    syms = declared_symbols("sam2d.h")
+    assert "sam2d_forward" in syms and "sam2d_gradient" in syms and "sam2d_create" in syms
+    missing = [s for s in syms if not hasattr(L, s)]
     assert not missing, missing
+    H = ctypes.CDLL(os.path.join(ROOT, "seisacomod2d_amd", "libsam2d_host.so"))
+    hsyms = declared_symbols("sam2d_modelling.h")
+    assert "sam2d_modelling_module" in hsyms and "write_shot_output" in hsyms
+    missing = [s for s in hsyms if not hasattr(H, s)]
+    assert not missing, missing
+    assert hasattr(H, "_Z16modelling_moduleiP5Geo2d")      # the C++ seam
 
 
 def test_version_and_variants():

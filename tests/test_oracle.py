@@ -176,6 +176,56 @@ def test_forward_mirror_symmetry(oracle):
     assert np.abs(s - s[::-1]).max() <= 1e-6 * np.abs(s).max()
 
 
+def _two_shot(oracle, g):
+    P = oracle.params(g["nx"], g["nz"], g["dx"], g["dz"], g["rec_len"], g["fd_dt"], g["real_dt"],
+                      g["fm"], 32, reset_quirk=1)
+    return oracle.forward(P, g["vp"], g["rho"], g["src_xz"], [g["rec_xz"], g["rec_xz"]])
+
+
+def test_two_shot_carryover_matches_reference_bitwise(oracle):
+    # The compiled reference ran two shots on one card; the output file it
+    # left holds shot 2, which started from shot 1's final state with only the
+    # first nnz cells (model column 0 of the sponge) cleared
+    # (cpumodelling.cpp:343-348, 367-374).  The oracle's reset_quirk mode gives
+    # the same bits; the fresh-state mode does not.
+    g = load_golden("two_shot_carryover")
+    assert np.array_equal(_two_shot(oracle, g)[1].view(np.int32), g["seis"].view(np.int32))
+    P = oracle.params(g["nx"], g["nz"], g["dx"], g["dz"], g["rec_len"], g["fd_dt"], g["real_dt"],
+                      g["fm"], 32)
+    fresh = oracle.forward(P, g["vp"], g["rho"], g["src_xz"], [g["rec_xz"], g["rec_xz"]])[1]
+    assert np.abs(fresh - g["seis"]).max() > 0.05 * np.abs(g["seis"]).max()
+
+
+@pytest.mark.parametrize("cells,fields,rejected", [
+    (None, 0b00110, True),      # clears px, pz everywhere, keeps vx, vz
+    (None, 0b11000, True),      # clears vx, vz everywhere, keeps px, pz
+    ("N", 31, True),            # fresh state
+    ("half", 31, True),         # half the grid
+    ("nnx", 31, False),         # nnx cells: still inside the damped sponge
+    (0, 31, False),             # nothing cleared
+])
+def test_two_shot_golden_rejects_planted_resets(oracle, cells, fields, rejected):
+    # Slips planted in the oracle's reset.  The ones that touch the interior
+    # change shot 2 and the reference's file rejects them.  Which sponge cells
+    # of column 0 are cleared cannot be seen at all: a state there is damped
+    # by the sponge (factor exp(-(0.05*31)^2) ~ 0.09 per step at column 0,
+    # more than 150 orders of magnitude before it reaches column 32) so the
+    # output is the same bits either way, and the reference's nnz-cell rule is
+    # pinned exactly as far as it is observable.
+    g = load_golden("two_shot_carryover")
+    nnz, nnx = g["nz"] + 64, g["nx"] + 64
+    n = {None: -1, "N": nnz * nnx, "half": nnz * nnx // 2, "nnx": nnx}.get(cells, cells)
+    if fields != 31:
+        n = nnz * nnx
+    oracle.set_reset_plant(n, fields)
+    try:
+        s2 = _two_shot(oracle, g)[1]
+    finally:
+        oracle.set_reset_plant()
+    same = np.array_equal(s2.view(np.int32), g["seis"].view(np.int32))
+    assert same != rejected
+
+
 def test_reset_quirk_only_matters_after_first_shot(oracle):
     P, vp, rho = _homog(oracle, T=0.05)
     rec = np.stack([np.arange(0, 810, 10.0), np.full(81, 100.0)], 1)

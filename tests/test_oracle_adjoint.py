@@ -92,3 +92,23 @@ def test_gradient_float_buoyancy_agrees_loosely(oracle):
           - oracle.functional(P, vp, rho - dr, src, recs, adj)) / 2
     an = np.sum(grho * dr)
     assert abs(fd - an) <= 1e-2 * abs(an)
+
+
+@pytest.mark.parametrize("k", [1, 7, 32, 10000])
+def test_checkpointed_gradient_equals_full_storage(oracle, k):
+    # Segments recomputed from checkpoints every k steps repeat the first
+    # sweep's steps exactly: the gradient is the same bits for every k
+    # (k = 1: every state a checkpoint; k >= fd_nt: one segment).
+    nx, nz = 36, 28
+    vp, rho = _small_case(nx=nx, nz=nz)
+    P = oracle.params(nx, nz, 10.0, 10.0, 0.16, 0.001, 0.002, 25.0, 8)
+    src, recs = _geometry(oracle, P, nx)
+    real_nt = oracle.shape(P)[1]
+    rng = np.random.default_rng(13)
+    adj = [rng.standard_normal((len(r), real_nt)).astype(np.float32) for r in recs]
+    full = oracle.gradient(P, vp, rho, src, recs, adj, padded=True)
+    ck = oracle.gradient(P, vp, rho, src, recs, adj, padded=True, ckpt_k=k)
+    for a, b in zip(full[:2], ck[:2]):
+        assert np.array_equal(a, b)
+    for f in ("kappa", "bu1", "bu2"):
+        assert np.array_equal(full[2][f], ck[2][f])
