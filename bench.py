@@ -1,33 +1,35 @@
 """Benchmark of the forward-modelling hot path on one MI355X.
 
-Default workload (the headline configuration): plain FP32 forward on a
-two-layer model at 5000 x 5000 cells, dx = dz = 10 m (interface at z = 2500 m;
-vp 1500/3000 m/s, rho 1000/2000 kg/m^3), dt = 1 ms, 4000 steps, sponge 50 cells
-(padded grid 5100 x 5100 = 26.01 M cells), 15 Hz Ricker, 64 shots evenly
-spaced along x at z = 20 m, 5000 surface receivers per shot, shots run one
-after another.  One bench "step" is one shot: the full 4000-step time loop
-over the padded grid plus its 5000-trace gather.  The metric is padded
-cell-updates per second.  Inputs are resident in HBM before the timed region
-(the context uploads the model and builds the coefficient fields once);
-seismograms are written to a device buffer.
+Default run, one JSON line:
+  * headline (value): plain FP32 forward on a two-layer model at 5000 x 5000
+    cells, dx = dz = 10 m (interface at z = 2500 m; vp 1500/3000 m/s, rho
+    1000/2000 kg/m^3), dt = 1 ms, 4000 steps, sponge 50 cells (padded grid
+    5100 x 5100 = 26.01 M cells), 15 Hz Ricker, 64 shots evenly spaced along x
+    at z = 20 m, 5000 surface receivers per shot, shots run one after another.
+    One bench "step" is one shot: the full 4000-step time loop over the padded
+    grid plus its 5000-trace gather.  The metric is padded cell-updates per
+    second.  Inputs are resident in HBM before the timed region (the context
+    uploads the model and builds the coefficient fields once); seismograms are
+    written to a device buffer.
+  * "batched": the salt-type 3201 x 1201 model (7.62 m, 0.5 ms, 8000 steps,
+    sponge 60), 64 shots with aperture windows 800-2400 cells: shots/s of the
+    batched launch against the per-shot loop on the same windows.
+  * "gradient": checkpointed adjoint gradient on the salt model, 32 windowed
+    shots, 8000 steps, checkpoints every 100 steps: seconds per shot and the
+    ratio to the batched forward and to the per-shot forward of the same shots.
+  * "cpu_baseline": the compiled reference (src_cpu) on this host.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-extra]
 
-Other workloads (each prints its own JSON line):
-    --workload batched    salt-type 3201 x 1201 model, 64 shots with aperture
-                          windows 800-2400 cells: shots/s of the batched launch
-                          against the per-shot loop on the same windows
-    --workload gradient   checkpointed adjoint gradient: 1201 x 601 smooth
-                          model with Gaussian anomalies, 16 shots, 3000 steps;
-                          and the salt model with 32 windowed shots, 8000
-                          steps, k = 50 / 100 / 200 under a 64 GiB budget
-                          (--quick: fewer shots): seconds per shot and the
-                          ratio to the forward
+Separate workloads (each prints its own JSON line):
+    --workload batched    the batched leg alone (--quick: 16 shots, 1 s)
+    --workload gradient   gradients of the 1201 x 601 anomaly model (16 shots,
+                          3000 steps, k = 100) and of the salt model (32
+                          windowed shots, 8000 steps, k = 50 / 100 / 200)
 
 --gpus N with N > 1 exits: this build runs on one GPU.
 """
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -37,6 +39,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+from seisacomod2d_amd import synthetic  # noqa: E402
 
 NX = NZ = 5000
 H = 10.0
@@ -48,64 +52,79 @@ NSHOTS = 64
 BYTES_PER_CELL = 44          # px, pz, vx, vz read + written (32) + KX, BX, BZ read (12)
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E peak, GB/s
 METRIC = "Gcell-updates/s + %HBM peak; shots/s batched vs plain; gradient s/shot"
+CPU_SHARE = 16               # host threads one GPU's job may use on the GPU hosts
 
 
 def model():
-    vp = np.full((NX, NZ), 1500, np.float32); vp[:, NZ // 2:] = 3000
-    rho = np.full((NX, NZ), 1000, np.float32); rho[:, NZ // 2:] = 2000
-    return vp, rho
+    return synthetic.two_layer(NX, NZ)
 
 
 def shots(sam):
-    rec = np.stack([np.arange(NX) * H, np.full(NX, 20.0)], 1).astype(np.float32)
-    out = []
-    for k in range(NSHOTS):
-        sx = 10.0 * int((k + 0.5) * NX / NSHOTS)
-        out.append(sam.Shot(sx, 20.0, rec))
-    return out
+    return [sam.Shot(sx, sz, rec) for sx, sz, rec in synthetic.surface_spread_shots(NX, H, NSHOTS)]
 
 
 class _Hip:
     """Minimal HIP runtime access for a device output buffer."""
 
     def __init__(self):
+        import ctypes
+        self.ct = ctypes
         self.L = ctypes.CDLL("libamdhip64.so")
 
     def malloc(self, nbytes):
-        p = ctypes.c_void_p()
-        rc = self.L.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes))
+        p = self.ct.c_void_p()
+        rc = self.L.hipMalloc(self.ct.byref(p), self.ct.c_size_t(nbytes))
         if rc != 0:
             raise RuntimeError(f"hipMalloc failed: {rc}")
         return p.value
 
     def free(self, p):
-        self.L.hipFree(ctypes.c_void_p(p))
+        self.L.hipFree(self.ct.c_void_p(p))
 
     def sync(self):
         self.L.hipDeviceSynchronize()
 
 
+def host_cpus():
+    """What this host offers: logical CPUs, the process's affinity set and the
+    physical cores among them (distinct (package, core) pairs)."""
+    logical = os.cpu_count() or 1
+    aff = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(logical))
+    phys = set()
+    for c in aff:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            phys.add((open(base + "physical_package_id").read().strip(),
+                      open(base + "core_id").read().strip()))
+        except OSError:
+            phys.add(("?", str(c)))
+    return {"logical_cpus": logical, "affinity_cpus": len(aff), "physical_cores_in_affinity": len(phys)}
+
+
 def cpu_baseline(vp, rho, seconds_target=15.0):
     """The reference's CPU path timed on this host: the compiled src_cpu build
     (oracle/_ref, "reference") when present, else the FP64 oracle in the
-    reference's loop structure ("port").  Cells/s of the time loop alone."""
-    threads = int(os.environ.get("BENCH_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    reference's loop structure ("port").  Cells/s of the time loop alone, at
+    min(CPU share, affinity) OpenMP threads."""
+    cpus = host_cpus()
+    threads = int(os.environ.get("BENCH_CPU_THREADS", min(CPU_SHARE, cpus["affinity_cpus"])))
     from oracle import refrun
     if refrun.available():
         try:
             # The reference fixes its sponge at 32 cells (cpumodelling.cpp:15):
             # its padded grid is 5064 x 5064.  Two runs, 20 and 320 steps,
             # timed by difference (setup and file I/O cancel).
+            sx = NX * H / 2
             per, steps, walls = refrun.time_per_step(
-                NX, NZ, H, vp, rho, f"{NX * H / 2:.6f},20.000000,{NX},0,{H:.6f},1",
-                0.02, 0.32, DT, FM, threads)
+                NX, NZ, H, vp, rho, f"{sx:.6f},20.000000,{NX},0,{H:.6f},1", 0.02, 0.32, DT, FM, threads)
             cells = (NX + 64) * (NZ + 64)
-            return {"value": round(cells / per / 1e9, 4), "unit": "Gcell-updates/s",
-                    "cores": threads, "kind": "reference",
-                    "sample": f"compiled reference src_cpu (FP64, MPI singleton, {threads} OpenMP "
-                              f"threads), 5000x5000 two-layer model at its fixed sponge of 32 "
-                              f"(5064x5064 padded), 1 shot, 5000 receivers; per-step time from two "
-                              f"runs of 20 and 320 steps ({walls[0]:.1f} s, {walls[1]:.1f} s)"}
+            return dict({"value": round(cells / per / 1e9, 4), "unit": "Gcell-updates/s",
+                         "cores": threads, "kind": "reference",
+                         "sample": f"compiled reference src_cpu (FP64, MPI singleton, {threads} OpenMP "
+                                   f"threads), 5000x5000 two-layer model at its fixed sponge of 32 "
+                                   f"(5064x5064 padded), 1 shot, 5000 receivers; per-step time from "
+                                   f"two runs of 20 and 320 steps ({walls[0]:.1f} s, {walls[1]:.1f} s)"},
+                        **cpus)
         except Exception as e:                      # fall back to the port
             print(f"# reference CPU run failed ({e}); timing the FP64 port", file=sys.stderr)
     from oracle import pyoracle
@@ -116,40 +135,29 @@ def cpu_baseline(vp, rho, seconds_target=15.0):
     per = t1 / 4
     n = max(4, min(400, int(seconds_target / max(per, 1e-6))))
     t, th = pyoracle.time_steps(P, vp, rho, n)
-    return {"value": round(cells * n / t / 1e9, 4), "unit": "Gcell-updates/s", "cores": th,
-            "kind": "port",
-            "sample": f"FP64 oracle, reference loop structure (9 sweeps/step), {n} steps of the "
-                      f"5100x5100 padded grid, {th} OpenMP threads, setup excluded"}
+    return dict({"value": round(cells * n / t / 1e9, 4), "unit": "Gcell-updates/s", "cores": th,
+                 "kind": "port",
+                 "sample": f"FP64 oracle, reference loop structure (9 sweeps/step), {n} steps of the "
+                           f"5100x5100 padded grid, {th} OpenMP threads, setup excluded"}, **cpus)
 
 
-def measured_traffic(name="k_step2_config1_pmc.txt"):
-    """HBM bytes per launch of the dominant kernel from rocprofv3 --pmc
-    (FETCH_SIZE x2 + WRITE_SIZE x1, the copy-kernel calibration), as recorded in
-    profiles/<name> ("traffic_bytes_per_launch: <n>")."""
-    path = os.path.join(ROOT, "profiles", name)
+def measured_traffic(kernel):
+    """HBM bytes per launch of the dominant kernel from rocprofv3 --pmc, as
+    recorded in profiles/<kernel>_config1_pmc.txt ("traffic_bytes_per_launch:
+    <n>" and the variant line "profiled: <variant> chunk <c> order <o>")."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_config1_pmc.txt")
     if not os.path.exists(path):
-        return None
+        return None, None
+    traffic, cfg = None, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
             try:
-                return float(line.split(":", 1)[1])
+                traffic = float(line.split(":", 1)[1])
             except ValueError:
-                return None
-    return None
-
-
-def copy_probe(nbytes=1 << 30):
-    """Calibration for rocprofv3's FETCH_SIZE / WRITE_SIZE: an elementwise
-    kernel (torch.neg) that reads nbytes and writes nbytes, run 3 times."""
-    import torch
-    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    for _ in range(3):
-        torch.neg(a, out=b)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
-                      "repeats": 3, "kernel": "torch.neg elementwise"}))
+                pass
+        elif line.startswith("profiled:"):
+            cfg = line.split(":", 1)[1].strip()
+    return traffic, cfg
 
 
 # --------------------------------------------------------------- config 1
@@ -206,26 +214,35 @@ def bench_plain(args, sam):
 
     cell_updates = float(cells) * m.fd_nt * args.steps
     gcells = cell_updates / wall / 1e9
-    launch_ms = k_ms / max(launches, 1)                     # one launch = spl time steps
-    # Algorithmic bytes: 44 B per cell-update (px, pz, vx, vz read and written,
-    # KX, BX, BZ read) times the cell-updates of one launch.  A launch that
-    # advances spl > 1 steps moves each cell's bytes once, so this rate can
-    # exceed the HBM peak; the DRAM traffic the counters measured is reported
-    # beside it.
-    bytes_per_launch = BYTES_PER_CELL * cells * spl
-    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    kname = {1: "k_step", 2: "k_step2", 3: "k_step3"}.get(spl, "k_step%d" % spl)
-    traffic = measured_traffic(f"{kname}_config1_pmc.txt")
+    # kernel_ms is HIP events around the time loop on the library's stream;
+    # one launch advances spl time steps
+    launch_ms = k_ms / max(launches, 1)
+    # Roofline of the dominant kernel.  What must cross HBM per launch is one
+    # read of the state and the coefficients and one write of the state:
+    # 44 B per cell, whatever the number of steps the launch advances (the
+    # steps between live in registers).  frac is that over the launch time,
+    # against the HBM peak.  effective_gbs counts 44 B per cell-update, the
+    # traffic a one-step-per-pass kernel would need for the same work.
+    hbm_bytes = BYTES_PER_CELL * cells
+    achieved = hbm_bytes / (launch_ms * 1e-3) / 1e9
+    kname = "k_step" if spl == 1 else "k_step%d" % spl
+    traffic, traffic_cfg = measured_traffic(kname)
+    this_cfg = f"{variant} chunk {chunk} order {tile_order}"
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": kname, "steps_per_launch": spl,
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-            "avg_launch_us": round(launch_ms * 1e3, 2)}
+            "algorithmic_bytes_per_launch": hbm_bytes,
+            "bytes_per_cell": BYTES_PER_CELL,
+            "avg_launch_us": round(launch_ms * 1e3, 2),
+            "effective_gbs": round(BYTES_PER_CELL * cells * spl / (launch_ms * 1e-3) / 1e9, 1),
+            "traffic_profiled_config": traffic_cfg,
+            "traffic_config_matches_run": traffic_cfg == this_cfg}
     if traffic:
         dram = traffic / (launch_ms * 1e-3) / 1e9
         roof["dram_gbs"] = round(dram, 1)
         roof["dram_frac"] = round(dram / HBM_PEAK_GBS, 4)
-    out = {
+        roof["traffic_over_algorithmic"] = round(traffic / hbm_bytes, 4)
+    return {
         "metric": METRIC,
         "value": round(gcells, 3),
         "unit": "Gcell-updates/s",
@@ -241,55 +258,21 @@ def bench_plain(args, sam):
         "config": {"workload": "config1_plain_forward_5000x5000_nt4000_npml50",
                    "grid": [NX, NZ], "padded": [m.nnx, m.nnz], "nt": m.fd_nt, "receivers": ng,
                    "shots_per_job": NSHOTS, "kernel_variant": variant, "tile_order": tile_order,
-                   "chunk": chunk,
-                   "hipgraph": used_graph,
-                   "profile_run": args.profile_steps > 0},
+                   "chunk": chunk, "hipgraph": used_graph, "profile_run": args.profile_steps > 0},
         "roofline": roof,
         "job_seconds_64_shots": round(wall / args.steps * NSHOTS, 2),
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(vp, rho)
-    print(json.dumps(out), flush=True)
 
 
 # --------------------------------------------------------------- config 2
-def salt_model(nx=3201, nz=1201, h=7.62, seed=2026):
-    """Water to z = 1500 m, then 1800 + 0.5 (z - 1500) m/s, a seeded random
-    polygon salt body at 4500 m/s; Gardner density rho = 310 vp^0.25."""
-    rng = np.random.default_rng(seed)
-    z = (np.arange(nz) * h)[None, :]
-    x = (np.arange(nx) * h)[:, None]
-    vp = np.where(z < 1500.0, 1500.0, 1800.0 + 0.5 * (z - 1500.0)) + 0.0 * x
-    cx, cz = nx * h * rng.uniform(0.35, 0.65), nz * h * rng.uniform(0.55, 0.7)
-    R = 0.18 * nx * h
-    ang = np.arctan2(z - cz, x - cx)
-    rad = R * np.ones_like(ang)
-    for k in range(1, 6):
-        rad += R * rng.uniform(-0.12, 0.12) * np.cos(k * ang + rng.uniform(0, 2 * np.pi))
-    rad *= np.where(z < cz, 1.0, 0.6)            # flatter base
-    vp = np.where(np.hypot(x - cx, z - cz) < rad, 4500.0, vp)
-    vp = vp.astype(np.float32)
-    rho = (310.0 * vp.astype(np.float64) ** 0.25).astype(np.float32)
-    return vp, rho
+def salt_shots(sam, n, seed=7):
+    sh, wins = synthetic.salt_shots(n, seed=seed)
+    return [sam.Shot(sx, sz, rec) for sx, sz, rec in sh], wins
 
 
-def salt_shots(sam, n, nx=3201, h=7.62, seed=7, zr=15.24):
-    rng = np.random.default_rng(seed)
-    out, wins = [], []
-    for k in range(n):
-        w = int(rng.integers(800, 2401))
-        cx = int((k + 0.5) * nx / n)
-        wx0 = min(max(cx - w // 2, 0), nx - w)
-        rec = np.stack([(wx0 + np.arange(w)) * h + 0.5 * h, np.full(w, zr)], 1).astype(np.float32)
-        out.append(sam.Shot((cx + 0.5) * h, zr, rec))
-        wins.append((wx0, w))
-    return out, wins
-
-
-def bench_batched(args, sam):
-    nshots = 16 if args.quick else 64
-    vp, rho = salt_model()
-    grid = sam.Grid(3201, 1201, 7.62, 7.62, 4.0 if not args.quick else 1.0, 0.0005, 0.0005, 15.0, npml=60)
+def bench_batched(args, sam, nshots=64, t_rec=4.0, batch_sizes=(0, 8, 16)):
+    vp, rho = synthetic.salt_model()
+    grid = sam.Grid(3201, 1201, 7.62, 7.62, t_rec, 0.0005, 0.0005, 15.0, npml=60)
     sh, wins = salt_shots(sam, nshots)
     cells = sum((w + 120) * (1201 + 120) for _, w in wins)
     with sam.Modeller(grid, vp, rho) as m:
@@ -303,7 +286,7 @@ def bench_batched(args, sam):
         print(f"# per-shot loop: {t_plain:.2f} s ({vplain})", file=sys.stderr, flush=True)
         # batched: variant tuned on the resident batch, for a few batch sizes
         t_batch, best = None, None
-        for mb in (args.max_batch,) if args.max_batch else (0, 8, 16):
+        for mb in batch_sizes:
             m.set_option("max_batch", mb)
             m.autotune(windows=wins, nshots=nshots, steps=6)
             m.forward_batched(sh, wins)            # warm: graphs captured and cached
@@ -315,36 +298,18 @@ def bench_batched(args, sam):
             if t_batch is None or t < t_batch:
                 t_batch, best = t, (mb, v)
         nt = m.fd_nt
-    out = {
-        "metric": METRIC,
-        "value": round(nshots / t_batch, 3), "unit": "shots/s", "n_gpus": 1,
-        "higher_is_better": True, "dtype": "f32", "data": "synthetic salt-type model (seeded)",
-        "config": {"workload": f"config2_batched_3201x1201_nt{nt}_{nshots}shots_windows800-2400",
-                   "batched_max_batch": best[0], "batched_variant": best[1], "plain_variant": vplain},
+    return {
+        "workload": f"config2_batched_3201x1201_nt{nt}_{nshots}shots_windows800-2400",
+        "shots": nshots, "batched_shots_per_s": round(nshots / t_batch, 3),
         "plain_shots_per_s": round(nshots / t_plain, 3),
         "speedup_vs_plain": round(t_plain / t_batch, 3),
         "batched_gcells": round(cells * nt / t_batch / 1e9, 2),
         "plain_gcells": round(cells * nt / t_plain / 1e9, 2),
+        "batched_max_batch": best[0], "batched_variant": best[1], "plain_variant": vplain,
     }
-    print(json.dumps(out), flush=True)
 
 
 # --------------------------------------------------------------- configs 3, 4
-def anomaly_model(nx=1201, nz=601, h=10.0, seed=3):
-    """Background vp = 2000 + 0.6 z plus six seeded Gaussian anomalies
-    (+-300 m/s, sigma 150 m); rho = 2000."""
-    rng = np.random.default_rng(seed)
-    z = (np.arange(nz) * h)[None, :]
-    x = (np.arange(nx) * h)[:, None]
-    bg = (2000.0 + 0.6 * z + 0.0 * x)
-    true = bg.copy()
-    for k in range(6):
-        cx, cz = rng.uniform(0.15, 0.85) * nx * h, rng.uniform(0.25, 0.75) * nz * h
-        true += rng.choice([-300.0, 300.0]) * np.exp(-((x - cx) ** 2 + (z - cz) ** 2) / (2 * 150.0 ** 2))
-    rho = np.full((nx, nz), 2000.0, np.float32)
-    return true.astype(np.float32), bg.astype(np.float32), rho
-
-
 def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False, adj_steps=2):
     """Gradient time against the forward modelling of the same shots: the
     batched forward (autotuned, the fastest forward path) and the per-shot
@@ -392,27 +357,28 @@ def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False, a
     return res
 
 
+def gradient_c4(args, sam, nshots=32, ks=(100,)):
+    from scipy.ndimage import gaussian_filter
+    vs, rs = synthetic.salt_model()
+    v4 = gaussian_filter(vs.astype(np.float64), 20.0).astype(np.float32)
+    grid4 = sam.Grid(3201, 1201, 7.62, 7.62, 4.0, 0.0005, 0.0005, 15.0, npml=60)
+    sh4, w4 = salt_shots(sam, nshots, seed=11)
+    return _grad_case(sam, grid4, vs, v4, rs, sh4, w4, list(ks),
+                      "config4_gradient_3201x1201_nt8000_windows", args.no_graph, args.adjoint_steps)
+
+
 def bench_gradient(args, sam):
     out = []
-    vtrue, v0, rho = anomaly_model()
+    vtrue, v0, rho = synthetic.anomaly_model()
     n3 = 4 if args.quick else 16
     grid = sam.Grid(1201, 601, 10.0, 10.0, 3.0, 0.001, 0.001, 10.0, npml=32)
     rec = np.stack([np.arange(1201) * 10.0, np.full(1201, 10.0)], 1).astype(np.float32)
     sh = [sam.Shot(10.0 * int((k + 0.5) * 1201 / n3), 10.0, rec) for k in range(n3)]
     out += _grad_case(sam, grid, vtrue, v0, rho, sh, None, [100], "config3_gradient_1201x601_nt3000",
                       args.no_graph, args.adjoint_steps)
-    # config 4: salt model, smoothed starting model, windowed shots
-    vs, rs = salt_model()
-    from scipy.ndimage import gaussian_filter
-    v4 = gaussian_filter(vs.astype(np.float64), 20.0).astype(np.float32)
-    n4 = 4 if args.quick else 32
-    grid4 = sam.Grid(3201, 1201, 7.62, 7.62, 4.0, 0.0005, 0.0005, 15.0, npml=60)
-    sh4, w4 = salt_shots(sam, n4, seed=11)
-    out += _grad_case(sam, grid4, vs, v4, rs, sh4, w4, [50, 100, 200] if not args.quick else [200],
-                      "config4_gradient_3201x1201_nt8000_windows", args.no_graph, args.adjoint_steps)
-    print(json.dumps({"metric": METRIC,
-                      "unit": "s/shot", "higher_is_better": False, "dtype": "f32",
-                      "results": out}), flush=True)
+    out += gradient_c4(args, sam, 4 if args.quick else 32, [200] if args.quick else [50, 100, 200])
+    return {"metric": METRIC, "unit": "s/shot", "higher_is_better": False, "dtype": "f32",
+            "results": out}
 
 
 def main():
@@ -420,7 +386,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8, help="timed shots")
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched and gradient legs of the default run")
     ap.add_argument("--variant", default="-1",
                     help="kernel variant: index or name (-1: autotuned)")
     ap.add_argument("--chunk", type=int, default=-1,
@@ -437,11 +405,6 @@ def main():
     ap.add_argument("--max-batch", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="profiling runs only: shorten each shot to this many time steps")
-    ap.add_argument("--stream-probe", action="store_true",
-                    help="memory-system ceiling of the kernels' access pattern on the 5100 x "
-                         "5100 grid (four data layouts, chunks 64 and 80), then exit")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="profiling calibration: one 1 GiB device-to-device copy, then exit")
     ap.add_argument("--tuning", action="store_true", help="print per-variant autotune times")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -449,19 +412,26 @@ def main():
               "(shots are independent, multi-GPU scaling is a shot queue per device).",
               file=sys.stderr)
         sys.exit(2)
-    if args.copy_probe:
-        copy_probe()
-        return
-    if args.stream_probe:
-        import subprocess
-        exe = os.path.join(ROOT, "seisacomod2d_amd", "stream_probe")
-        if not os.path.exists(exe):
-            sys.exit("stream_probe is not built (python -m seisacomod2d_amd.build)")
-        for chunk in (64, 80):
-            subprocess.run([exe, str(chunk), "4"], check=True)
-        return
     import seisacomod2d_amd as sam
-    {"plain": bench_plain, "batched": bench_batched, "gradient": bench_gradient}[args.workload](args, sam)
+    if args.workload == "batched":
+        mbs = (args.max_batch,) if args.max_batch else (0, 8, 16)
+        res = bench_batched(args, sam, 16 if args.quick else 64, 1.0 if args.quick else 4.0, mbs)
+        print(json.dumps(dict({"metric": METRIC, "value": res["batched_shots_per_s"],
+                               "unit": "shots/s", "n_gpus": 1, "higher_is_better": True,
+                               "dtype": "f32", "data": "synthetic salt-type model (seeded)"},
+                              **res)), flush=True)
+        return
+    if args.workload == "gradient":
+        print(json.dumps(bench_gradient(args, sam)), flush=True)
+        return
+    out = bench_plain(args, sam)
+    if not args.no_extra and args.profile_steps <= 0:
+        out["batched"] = bench_batched(args, sam)
+        g = gradient_c4(args, sam)[0]
+        out["gradient"] = g
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(*model())
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
