@@ -1,4 +1,6 @@
-"""ctypes front end of the FP64 CPU oracle (oracle/liboracle.so).
+"""ctypes front end of the CPU checkers in oracle/liboracle.so: the FP64
+oracle of the reference (sam2d_oracle.c) and the FP32 model of the product's
+kernels (sam2d_fp32.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
 by bench.py's cpu_baseline leg, always as the checker.  The product package
@@ -254,3 +256,31 @@ def set_reset_plant(cells=-1, fields=31):
     L = lib()
     L.orc_set_reset_plant.argtypes = [ctypes.c_long, ctypes.c_int]
     L.orc_set_reset_plant(int(cells), int(fields))
+
+
+def fp32_forward(fields, fd_nt, nts, amp, src_cell, gcells, dt_factor, real_nt, state=None):
+    """The product kernels' FP32 arithmetic on the CPU (sam2d_fp32.c).
+    fields: KX, KZ, BX, BZ (nnx, nnz) and prof_x (4, nnx), prof_z (4, nnz) as
+    the library builds them; amp: source increments; src_cell, gcells: padded
+    cell indices ix*nnz + iz.  state: (px, pz, vx, vz) to start from (updated
+    in place), default zero.  Returns (seis (ng, real_nt), state)."""
+    KX = _f32(fields["KX"])
+    nnx, nnz = KX.shape
+    f = {k: _f32(fields[k]) for k in ("KZ", "BX", "BZ", "prof_x", "prof_z")}
+    if state is None:
+        state = tuple(np.zeros((nnx, nnz), np.float32) for _ in range(4))
+    g = np.ascontiguousarray(gcells, np.int64)
+    seis = np.zeros((len(g), real_nt), np.float32)
+    amp = _f32(np.asarray(amp).ravel() if len(amp) else np.zeros(1))
+    L = lib()
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.f32_forward.argtypes = [ctypes.c_int, ctypes.c_int] + [fp] * 6 + [ctypes.c_int, ctypes.c_int, fp,
+                              ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.POINTER(ctypes.c_long)] + [fp] * 5
+    rc = L.f32_forward(nnx, nnz, _ptr(KX), _ptr(f["KZ"]), _ptr(f["BX"]), _ptr(f["BZ"]),
+                       _ptr(f["prof_x"]), _ptr(f["prof_z"]), int(fd_nt), int(nts), _ptr(amp),
+                       int(src_cell), int(dt_factor), int(real_nt), len(g), _ptr(g, ctypes.c_long),
+                       _ptr(seis), *[_ptr(a) for a in state])
+    if rc != 0:
+        raise RuntimeError(f"f32_forward failed: {rc}")
+    return seis, state

@@ -223,13 +223,54 @@ int tile_chunk(const sam2d_ctx *ctx, int variant)
     return ctx->chunk > 0 ? ctx->chunk : march_aligned(chunk, 6 * steps - 3);
 }
 
+// Descriptor of one shot for `variant` with march length `chunk`, its tiles
+// numbered from tile_begin; the device buffers (state, amp, gcell, seis,
+// band) are left for the caller.  Rows spanned by the receivers: the
+// multi-step kernels store the intermediate pressure of these rows into a
+// band buffer (two parities x BAND_SLICES slices of band_h x nnx) for the
+// deferred gather.
+void describe_shot(const ShotPlan &p, int variant, int chunk, int tile_begin, int nts, int real_nt,
+                   ShotDev &d)
+{
+    int waves, vchunk;
+    variant_shape(variant, &waves, &vchunk);
+    const DeviceDomain *dom = p.dom;
+    const int nnz = dom->host.nnz, nnx = dom->host.nnx;
+    int rlo = 1, rhi = 0;
+    for (int g = 0; g < p.ng; g++) {
+        const int row = p.gcell[g] % nnz;
+        if (g == 0 || row < rlo) rlo = row;
+        if (g == 0 || row > rhi) rhi = row;
+    }
+    std::memset(&d, 0, sizeof d);
+    d.nnz = nnz; d.nnx = nnx;
+    tile_shape(nnz, nnx, waves, chunk, variant_own_rows(variant), d.ngroups, d.nchunks);
+    d.chunk = chunk;
+    d.tile_begin = tile_begin;
+    d.src_x = p.src_x; d.src_z = p.src_z;
+    d.nts = nts; d.ng = p.ng; d.real_nt = real_nt;
+    ones_range(dom->host.prof_x, nnx, d.xin_lo, d.xin_hi);
+    d.rrow_lo = rlo; d.rrow_hi = rhi; d.band_h = p.ng > 0 ? rhi - rlo + 1 : 0;
+    d.KX = dom->KX; d.KZ = dom->KZ; d.BX = dom->BX; d.BZ = dom->BZ;
+    d.prof_x = dom->prof_x; d.prof_z = dom->prof_z;
+    d.ncell = (size_t)nnz * nnx;
+}
+
+// 32-bit byte offsets of the multi-step kernels: 16 N bytes of
+// cell-interleaved state, 4 N per field.
+int check_multistep_size(const sam2d_ctx *ctx, int variant, size_t ncell)
+{
+    if (variant_steps(variant) > 1 && ncell >= ((size_t)1 << (ctx->state_aos ? 28 : 30)))
+        return fail(SAM2D_ERR_PARAM, "grid of %zu cells too large for a multi-step variant "
+                    "(32-bit byte offsets); choose a single-step variant", ncell);
+    return SAM2D_OK;
+}
+
 // Make `plans` resident: buffers, descriptors, zeroed state and seismograms.
 int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *ntiles_out,
                 int nslots)
 {
-    int waves, chunk;
-    variant_shape(variant, &waves, &chunk);
-    chunk = tile_chunk(ctx, variant);
+    const int chunk = tile_chunk(ctx, variant);
     const int ns = (int)plans.size();
     if ((int)ctx->bufs.size() < ns) ctx->bufs.resize(ns);
     std::vector<ShotDev> desc(ns);
@@ -238,47 +279,22 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
     for (int s = 0; s < ns; s++) {
         ShotPlan &p = plans[s];
         DeviceDomain *dom = p.dom;
-        const int nnz = dom->host.nnz, nnx = dom->host.nnx;
-        const size_t N = (size_t)nnz * nnx;
-        // 32-bit byte offsets: 16 N bytes of cell-interleaved state, 4 N per field
-        if (variant_steps(variant) > 1 && N >= ((size_t)1 << (ctx->state_aos ? 28 : 30)))
-            return fail(SAM2D_ERR_PARAM, "grid of %zu cells too large for a multi-step variant "
-                        "(32-bit byte offsets); choose a single-step variant", N);
+        const int nnx = dom->host.nnx;
+        const size_t N = (size_t)dom->host.nnz * nnx;
+        SAM2D_TRY(check_multistep_size(ctx, variant, N));
         if (!ctx->bufs[s]) ctx->bufs[s].reset(new ShotBuffers());
         ShotBuffers &b = *ctx->bufs[s];
         SAM2D_TRY(b.reserve(N, nslots, nts, p.ng, (size_t)p.ng * real_nt));
         SAM2D_TRY(upload(b.amp, p.amp.data(), sizeof(float) * nts, ctx->stream));
         SAM2D_TRY(upload(b.gcell, p.gcell.data(), sizeof(int) * p.ng, ctx->stream));
-        // Rows spanned by the receivers: the multi-step kernels store the
-        // intermediate pressure of these rows into a band buffer (two
-        // parities x BAND_SLICES slices of band_h x nnx) for the deferred
-        // gather.
-        int rlo = 1, rhi = 0;
-        for (int g = 0; g < p.ng; g++) {
-            const int row = p.gcell[g] % nnz;
-            if (g == 0 || row < rlo) rlo = row;
-            if (g == 0 || row > rhi) rhi = row;
-        }
-        const int band_h = p.ng > 0 ? rhi - rlo + 1 : 0;
-        SAM2D_TRY(b.reserve_band((size_t)2 * BAND_SLICES * band_h * nnx));
+        ShotDev &d = desc[s];
+        describe_shot(p, variant, chunk, tiles, nts, real_nt, d);
+        tiles += d.ngroups * d.nchunks;
+        SAM2D_TRY(b.reserve_band((size_t)2 * BAND_SLICES * d.band_h * nnx));
         HIP_TRY(launch_fill(b.slot(0), 4 * N, 0.0f, ctx->stream));
         if (p.ng) HIP_TRY(launch_fill(b.seis, (size_t)p.ng * real_nt, 0.0f, ctx->stream));
-        ShotDev &d = desc[s];
-        std::memset(&d, 0, sizeof d);
-        d.nnz = nnz; d.nnx = nnx;
-        tile_shape(nnz, nnx, waves, chunk, variant_own_rows(variant), d.ngroups, d.nchunks);
-        d.chunk = chunk;
-        d.tile_begin = tiles;
-        tiles += d.ngroups * d.nchunks;
-        d.src_x = p.src_x; d.src_z = p.src_z;
-        d.nts = nts; d.ng = p.ng; d.real_nt = real_nt;
-        ones_range(dom->host.prof_x, nnx, d.xin_lo, d.xin_hi);
-        d.rrow_lo = rlo; d.rrow_hi = rhi; d.band_h = band_h;
         d.amp = b.amp;
-        d.KX = dom->KX; d.KZ = dom->KZ; d.BX = dom->BX; d.BZ = dom->BZ;
-        d.prof_x = dom->prof_x; d.prof_z = dom->prof_z;
         d.state = b.state;
-        d.ncell = N;
         d.gcell = b.gcell; d.seis = b.seis;
         d.band = b.band;
         p.seis_dev = b.seis;

@@ -146,6 +146,12 @@ int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int n0, i
 int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int ngroups, int nshots, int ntiles,
                      int variant, int n0, int n1, int max_ng, int start_slot, int *end_slot);
 void tile_shape(int nnz, int nnx, int waves, int chunk, int own_rows, int &ngroups, int &nchunks);
+// Descriptor of one shot for `variant` with march length `chunk`, tiles from
+// tile_begin; the device buffers are left for the caller to set.
+void describe_shot(const ShotPlan &p, int variant, int chunk, int tile_begin, int nts, int real_nt,
+                   ShotDev &d);
+// SAM2D_ERR_PARAM for a multi-step variant on a grid beyond its 32-bit offsets.
+int check_multistep_size(const sam2d_ctx *ctx, int variant, size_t ncell);
 }  // namespace sam2d
 
 void sam2d_adjoint_release(sam2d_ctx *ctx);

@@ -1,7 +1,8 @@
 // sam2d_setup.h -- host-side construction of one padded modelling domain:
 // time axis, sponge profiles, Ricker wavelet, staggered coefficients.
 // Computed in FP64 where the reference computes in FP64, in FP32 where it
-// computes in FP32, and rounded once to the FP32 fields the kernels read.
+// computes in FP32, and rounded once, without bias, to the FP32 fields the
+// kernels read.
 #pragma once
 #include <vector>
 #include "../../include/sam2d.h"

@@ -63,27 +63,37 @@ def load_golden(name):
 FLOOR = 1e-3
 
 
-def trace_errors(test, ref):
+# The reference's two-shot carry-over (cpumodelling.cpp:343-348) hands shot 2
+# shot 1's late wavefield.  Its FP32 error is set by shot 1's direct arrival
+# (about 1e-6 of that arrival, absolute), and the traces of shot 2 near shot
+# 1's source record little else than that weak late field (down to 4e-4 of
+# the gather's largest trace): their relative error reaches 1e-4 in any FP32
+# state, whatever the arithmetic of the update.  Those comparisons hold
+# traces below 2 % of the largest trace to 1e-5 of that 2 %.
+CARRYOVER_FLOOR = 0.02
+
+
+def trace_errors(test, ref, floor=FLOOR):
     test = np.asarray(test, np.float64)
     ref = np.asarray(ref, np.float64)
     err = np.linalg.norm(test - ref, axis=-1)
     nrm = np.linalg.norm(ref, axis=-1)
     mx = nrm.max() if nrm.size else 0.0
-    scale = np.maximum(nrm, FLOOR * mx)
+    scale = np.maximum(nrm, floor * mx)
     scale = np.where(scale > 0, scale, 1.0)
     return err / scale
 
 
-def floored_traces(ref):
+def floored_traces(ref, floor=FLOOR):
     """Number of traces held to the gather-relative floor instead of their own norm."""
     nrm = np.linalg.norm(np.asarray(ref, np.float64), axis=-1)
-    return int(np.sum(nrm < FLOOR * nrm.max())) if nrm.size else 0
+    return int(np.sum(nrm < floor * nrm.max())) if nrm.size else 0
 
 
-def assert_seis_close(test, ref, tol, what=""):
+def assert_seis_close(test, ref, tol, what="", floor=FLOOR):
     assert test.shape == ref.shape, (test.shape, ref.shape)
     assert np.all(np.isfinite(test)), f"{what}: non-finite samples"
-    e = trace_errors(test, ref)
+    e = trace_errors(test, ref, floor)
     worst = int(np.argmax(e)) if e.size else 0
     assert e.size == 0 or e.max() <= tol, (
         f"{what}: worst trace {worst} relative L2 error {e.max():.3e} > {tol:g}")

@@ -10,7 +10,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ROOT, assert_seis_close, load_golden
+from conftest import CARRYOVER_FLOOR, ROOT, assert_seis_close, load_golden
 from test_host_cli import _write_card
 
 pytestmark = pytest.mark.gpu
@@ -50,5 +50,5 @@ def test_dropin_two_shot_carryover(tmp_path):
     card = _write_card(d, g, 0, "\n".join(lines))
     _run(d, card, {"SAM2D_RESET_QUIRK": "1"})
     out = np.fromfile(os.path.join(d, "card_seismogram.bin"), np.float32).reshape(g["seis"].shape)
-    e = assert_seis_close(out, g["seis"], 1e-5, "drop-in two shots")
+    e = assert_seis_close(out, g["seis"], 1e-5, "drop-in two shots", CARRYOVER_FLOOR)
     print(f"[gpu] drop-in two-shot carry-over: worst trace error {e.max():.2e}", flush=True)

@@ -72,17 +72,61 @@ def test_coefficients_match_oracle_setup(oracle, case, npml):
     np.testing.assert_array_equal(c["prof_z"][2], o["damp1b"])
     np.testing.assert_array_equal(c["prof_z"][3], o["damp1b1"])
     # The kernels form kappaw2 = KX * damp2a1[ix] etc.  Where the profile is 1
-    # (interior) this is the reference's FP64 coefficient rounded once to FP32,
-    # bit for bit; in the sponge it is within 2 float ulps.
+    # (interior) KX is the reference's FP64 coefficient rounded to one of the
+    # two floats around it (unbiased rounding, test below); in the sponge the
+    # product is within 2 float ulps.
     px1, pz1 = c["prof_x"][1][:, None], c["prof_z"][1][None, :]
     bx1, bz1 = c["prof_x"][3][:, None], c["prof_z"][3][None, :]
     for ours, ref in [(c["KX"] * px1, o["kappaw2"]), (c["KZ"] * pz1, o["kappaw1"]),
                       (c["BX"] * bx1, o["buw2"]), (c["BZ"] * bz1, o["buw1"])]:
-        r32 = ref.astype(np.float32)
         inner = (px1 == 1) & (pz1 == 1) & (bx1 == 1) & (bz1 == 1)
         inner = np.broadcast_to(inner, ref.shape)
-        np.testing.assert_array_equal(ours[inner], r32[inner])
+        lo, hi = _bracket(ref[inner])
+        v = ours[inner]
+        assert np.all((v == lo) | (v == hi))
         np.testing.assert_allclose(ours, ref, rtol=2.5e-7, atol=0)
+
+
+def _bracket(t):
+    """The two floats around each FP64 value (equal where t is a float)."""
+    t = np.asarray(t, np.float64)
+    n = t.astype(np.float32)
+    lo = np.where(n.astype(np.float64) > t, np.nextafter(n, np.float32(-np.inf)), n)
+    hi = np.where(lo.astype(np.float64) == t, lo, np.nextafter(lo, np.float32(np.inf)))
+    return lo, hi
+
+
+@pytest.mark.parametrize("vp,rho", [(1500.0, 310.0 * 1500.0 ** 0.25), (2750.5, 1333.3)])
+def test_coefficient_rounding_is_unbiased(oracle, vp, rho):
+    # A homogeneous model: every interior cell has the same FP64 coefficient
+    # t.  Round-to-nearest would store the same float everywhere (a
+    # systematic error of up to 2^-24 relative); the fields hold the upper
+    # neighbour of t in a fraction (t - lo)/(hi - lo) of the cells, so the
+    # mean stored value is t.  Binomial bounds at 5 sigma.
+    nx, nz = 300, 200
+    grid = sam.Grid(nx, nz, 7.62, 7.62, 0.1, 0.0005, 0.0005, 15.0, 32)
+    vpa = np.full((nx, nz), vp, np.float32)
+    rhoa = np.full((nx, nz), rho, np.float32)
+    c = sam.coefficients(grid, vpa, rhoa)
+    P = oracle.params(nx, nz, 7.62, 7.62, 0.1, 0.0005, 0.0005, 15.0, 32)
+    o = oracle.setup(P, vpa, rhoa)
+    sl = (slice(40, -40), slice(40, -40))           # interior, away from the edge rows
+    for ours, ref in [(c["KX"], o["kappaw2"]), (c["BX"], o["buw2"]), (c["BZ"], o["buw1"])]:
+        t = ref[sl]
+        assert np.all(t == t.flat[0])
+        lo, hi = _bracket(t.flat[0])
+        assert lo != hi
+        frac = (t.flat[0] - float(lo)) / (float(hi) - float(lo))
+        v = ours[sl]
+        assert np.all((v == lo) | (v == hi))
+        n = v.size
+        up = np.count_nonzero(v == hi)
+        assert abs(up - frac * n) <= 5 * np.sqrt(n * frac * (1 - frac)) + 1, (up, frac * n)
+        mean_err = (v.astype(np.float64).mean() - t.flat[0]) / (float(hi) - float(lo))
+        assert abs(mean_err) <= 5 * 0.5 / np.sqrt(n)
+        # neighbouring cells are not all rounded alike (no stripes along z or x)
+        assert 0 < np.count_nonzero(v[:, 0] == hi) < v.shape[0]
+        assert 0 < np.count_nonzero(v[0, :] == hi) < v.shape[1]
 
 
 def test_source_increments_match_reference_formula(oracle):
