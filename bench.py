@@ -69,7 +69,7 @@ class _Hip:
     def __init__(self):
         import ctypes
         self.ct = ctypes
-        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L = ctypes.CDLL("libamdhip64.so.7")     # the runtime libsam2d.so links
 
     def malloc(self, nbytes):
         p = self.ct.c_void_p()
@@ -310,10 +310,19 @@ def bench_batched(args, sam, nshots=64, t_rec=4.0, batch_sizes=(0, 8, 16)):
 
 
 # --------------------------------------------------------------- configs 3, 4
-def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False, adj_steps=2):
+def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False, adj_steps=2,
+               fixed=None):
     """Gradient time against the forward modelling of the same shots: the
     batched forward (autotuned, the fastest forward path) and the per-shot
-    loop."""
+    loop.  fixed: (variant, chunk, tile order) to use instead of the tuner
+    (profiling runs)."""
+    def tune(m, **kw):
+        if fixed is None:
+            m.autotune(**kw)
+        else:
+            m.set_option("variant", fixed[0])
+            m.set_option("chunk", fixed[1])
+            m.set_option("tile_order", fixed[2])
     res = []
     with sam.Modeller(grid, vtrue, rho) as mt:
         if no_graph:
@@ -326,14 +335,14 @@ def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False, a
         # per-shot loop with a variant tuned on one shot; one untimed batched
         # pass sets up every window's coefficients first, as the timed
         # batched and gradient calls find them
-        m.autotune(windows=wins[:1] if wins else None, nshots=1, steps=6)
+        tune(m, windows=wins[:1] if wins else None, nshots=1, steps=6)
         m.forward_batched(sh, wins)
         m.forward_windows(sh[:1], wins[:1]) if wins else m.forward(sh[:1])   # warm
         t0 = time.perf_counter()
         m.forward_windows(sh, wins) if wins else m.forward(sh)
         t_plain = time.perf_counter() - t0
         # batched forward and gradient with a variant tuned on the batch
-        m.autotune(windows=wins, nshots=len(sh), steps=6)
+        tune(m, windows=wins, nshots=len(sh), steps=6)
         d = m.forward_batched(sh, wins)                          # warm
         t0 = time.perf_counter()
         d = m.forward_batched(sh, wins)
@@ -357,6 +366,13 @@ def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False, a
     return res
 
 
+def _fixed(args, sam):
+    if str(args.variant) in ("-1", ""):
+        return None
+    v = args.variant if str(args.variant).lstrip("-").isdigit() else sam.variants().index(args.variant)
+    return (int(v), max(args.chunk, 0), max(args.tile_order, 0))
+
+
 def gradient_c4(args, sam, nshots=32, ks=(100,)):
     from scipy.ndimage import gaussian_filter
     vs, rs = synthetic.salt_model()
@@ -364,7 +380,8 @@ def gradient_c4(args, sam, nshots=32, ks=(100,)):
     grid4 = sam.Grid(3201, 1201, 7.62, 7.62, 4.0, 0.0005, 0.0005, 15.0, npml=60)
     sh4, w4 = salt_shots(sam, nshots, seed=11)
     return _grad_case(sam, grid4, vs, v4, rs, sh4, w4, list(ks),
-                      "config4_gradient_3201x1201_nt8000_windows", args.no_graph, args.adjoint_steps)
+                      "config4_gradient_3201x1201_nt8000_windows", args.no_graph, args.adjoint_steps,
+                      _fixed(args, sam))
 
 
 def bench_gradient(args, sam):
@@ -374,9 +391,13 @@ def bench_gradient(args, sam):
     grid = sam.Grid(1201, 601, 10.0, 10.0, 3.0, 0.001, 0.001, 10.0, npml=32)
     rec = np.stack([np.arange(1201) * 10.0, np.full(1201, 10.0)], 1).astype(np.float32)
     sh = [sam.Shot(10.0 * int((k + 0.5) * 1201 / n3), 10.0, rec) for k in range(n3)]
-    out += _grad_case(sam, grid, vtrue, v0, rho, sh, None, [100], "config3_gradient_1201x601_nt3000",
-                      args.no_graph, args.adjoint_steps)
-    out += gradient_c4(args, sam, 4 if args.quick else 32, [200] if args.quick else [50, 100, 200])
+    if not args.c4_only:
+        out += _grad_case(sam, grid, vtrue, v0, rho, sh, None, [100],
+                          "config3_gradient_1201x601_nt3000", args.no_graph, args.adjoint_steps,
+                          _fixed(args, sam))
+    ks = [int(k) for k in args.ckpt_k.split(",")] if args.ckpt_k else (
+        [200] if args.quick else [50, 100, 200])
+    out += gradient_c4(args, sam, args.shots or (4 if args.quick else 32), ks)
     return {"metric": METRIC, "unit": "s/shot", "higher_is_better": False, "dtype": "f32",
             "results": out}
 
@@ -403,6 +424,9 @@ def main():
     ap.add_argument("--workload", default="plain", choices=["plain", "batched", "gradient"])
     ap.add_argument("--quick", action="store_true", help="batched/gradient: fewer shots")
     ap.add_argument("--max-batch", type=int, default=0)
+    ap.add_argument("--c4-only", action="store_true", help="gradient workload: the salt model only")
+    ap.add_argument("--ckpt-k", default="", help="gradient workload: checkpoint intervals, e.g. 100")
+    ap.add_argument("--shots", type=int, default=0, help="gradient workload: salt-model shots")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="profiling runs only: shorten each shot to this many time steps")
     ap.add_argument("--tuning", action="store_true", help="print per-variant autotune times")
