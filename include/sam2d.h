@@ -134,6 +134,24 @@ int sam2d_forward_dev(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
 int sam2d_forward_batched(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
                           const int *windows, float *seis);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sam2d_forward_dev for one shot on the full model, with
+ * every working buffer (the shot's descriptor, source increments, receiver
+ * cells, receiver band and both state slots) in one static device array of
+ * SAM2D_NOALLOC_SCRATCH_BYTES that each GPU holds from the library's first
+ * use until the process ends.  seis_dev (caller-owned device memory) receives
+ * nrec * real_nt floats, the same bits as sam2d_forward_dev gives.  The shot
+ * needs 32 bytes per padded cell plus its band and receiver lists, so the
+ * padded grid is limited to about 6 M cells (the 3201 x 1201 model with a
+ * 60-cell sponge fits; 5000 x 5000 does not): larger shots return
+ * SAM2D_ERR_PARAM.  Calls no hipMalloc or hipFree and captures no graph.
+ * Every call on a GPU shares that GPU's array, which keeps the last call's
+ * contents: calls that overlap on one GPU (from several threads or
+ * contexts) race. */
+#define SAM2D_NOALLOC_SCRATCH_BYTES (192u << 20)
+int sam2d_forward_noalloc(sam2d_ctx *ctx, const sam2d_shot *shot, float *seis_dev);
+
 /* The per-shot loop over cropped windows (the plain baseline that
  * sam2d_forward_batched is compared with): same arguments and results. */
 int sam2d_forward_windows(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,

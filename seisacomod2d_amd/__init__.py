@@ -74,6 +74,7 @@ def lib():
         L.sam2d_get_dims.argtypes = [vp, P(_Dims)]
         L.sam2d_forward.argtypes = [vp, P(_Shot), ctypes.c_int, f32p]
         L.sam2d_forward_dev.argtypes = [vp, P(_Shot), ctypes.c_int, vp]
+        L.sam2d_forward_noalloc.argtypes = [vp, P(_Shot), vp]
         L.sam2d_forward_batched.argtypes = [vp, P(_Shot), ctypes.c_int, i32p, f32p]
         L.sam2d_forward_windows.argtypes = [vp, P(_Shot), ctypes.c_int, i32p, f32p]
         L.sam2d_gradient.argtypes = [vp, P(_Shot), ctypes.c_int, i32p, f32p, ctypes.c_int,
@@ -241,6 +242,11 @@ class Modeller:
         """Per-shot loop with the seismograms written to a device buffer."""
         sc = _shots_c(shots)
         _check(lib().sam2d_forward_dev(self._h, sc, len(shots), ctypes.c_void_p(seis_dev_ptr)))
+
+    def forward_noalloc(self, shot: Shot, seis_dev_ptr: int):
+        """One shot on the full model with no device allocation (working memory
+        in the library's static scratch array), seismogram to a device buffer."""
+        _check(lib().sam2d_forward_noalloc(self._h, _shots_c([shot]), ctypes.c_void_p(seis_dev_ptr)))
 
     def forward_windows(self, shots, windows) -> List[np.ndarray]:
         """Per-shot loop, each shot on its aperture window (first column, width)."""

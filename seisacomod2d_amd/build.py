@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
 LIB_SOURCES = ["sam2d_kernels.hip", "sam2d_kernelsn.hip", "sam2d_adjoint_kernels.hip", "sam2d_api.cpp",
-               "sam2d_adjoint.cpp", "sam2d_setup.cpp"]
+               "sam2d_adjoint.cpp", "sam2d_setup.cpp", "sam2d_noalloc.hip"]
 HOST_SOURCES = ["jobcard.cpp", "geometry.cpp", "model_io.cpp", "modelling_module.cpp"]
 
 

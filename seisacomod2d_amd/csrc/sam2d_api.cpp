@@ -659,6 +659,8 @@ int sam2d_create(const sam2d_grid *g, const float *vp, const float *rho, sam2d_c
     HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
     ctx->num_cu = prop.multiProcessorCount;
     ctx->default_variant = default_variant_index();
+    void *scratch_ptr = nullptr;                   // sam2d_forward_noalloc's array on this device
+    SAM2D_TRY(noalloc_scratch(ctx->device, &scratch_ptr));
     DeviceDomain *dom = nullptr;
     SAM2D_TRY(get_domain(ctx.get(), 0, g->nx, &dom));
     *out = ctx.release();

@@ -152,6 +152,9 @@ void describe_shot(const ShotPlan &p, int variant, int chunk, int tile_begin, in
                    ShotDev &d);
 // SAM2D_ERR_PARAM for a multi-step variant on a grid beyond its 32-bit offsets.
 int check_multistep_size(const sam2d_ctx *ctx, int variant, size_t ncell);
+// Device address of the no-allocation path's scratch array on `device`
+// (sam2d_noalloc.hip), looked up once per device.
+int noalloc_scratch(int device, void **out);
 }  // namespace sam2d
 
 void sam2d_adjoint_release(sam2d_ctx *ctx);
