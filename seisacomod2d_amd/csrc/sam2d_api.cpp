@@ -67,7 +67,7 @@ int upload(void *dst, const void *src, size_t bytes, hipStream_t s)
 DeviceDomain::~DeviceDomain()
 {
     dev_free(KX); if (KZ != KX) dev_free(KZ);
-    dev_free(BX); dev_free(BZ); dev_free(prof_x); dev_free(prof_z);
+    dev_free(BX); dev_free(BZ); dev_free(B); dev_free(prof_x); dev_free(prof_z);
 }
 
 ShotBuffers::~ShotBuffers() { release(); }
@@ -131,6 +131,7 @@ int get_domain(sam2d_ctx *ctx, int wx0, int w, DeviceDomain **out)
     SAM2D_TRY(dev_alloc((void **)&d->KX, N * 4));
     SAM2D_TRY(dev_alloc((void **)&d->BX, N * 4));
     SAM2D_TRY(dev_alloc((void **)&d->BZ, N * 4));
+    SAM2D_TRY(dev_alloc((void **)&d->B, N * 4));
     if (h.kz_alias) d->KZ = d->KX;
     else SAM2D_TRY(dev_alloc((void **)&d->KZ, N * 4));
     SAM2D_TRY(dev_alloc((void **)&d->prof_x, h.prof_x.size() * 4));
@@ -139,12 +140,14 @@ int get_domain(sam2d_ctx *ctx, int wx0, int w, DeviceDomain **out)
     if (!h.kz_alias) SAM2D_TRY(upload(d->KZ, h.KZ.data(), N * 4, ctx->stream));
     SAM2D_TRY(upload(d->BX, h.BX.data(), N * 4, ctx->stream));
     SAM2D_TRY(upload(d->BZ, h.BZ.data(), N * 4, ctx->stream));
+    SAM2D_TRY(upload(d->B, h.B.data(), N * 4, ctx->stream));
     SAM2D_TRY(upload(d->prof_x, h.prof_x.data(), h.prof_x.size() * 4, ctx->stream));
     SAM2D_TRY(upload(d->prof_z, h.prof_z.data(), h.prof_z.size() * 4, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     // The FP32 host copies are no longer needed; kappa stays for the source.
     d->host.KX = std::vector<float>(); d->host.KZ = std::vector<float>();
     d->host.BX = std::vector<float>(); d->host.BZ = std::vector<float>();
+    d->host.B = std::vector<float>();
     *out = d.get();
     ctx->domains[key] = std::move(d);
     return SAM2D_OK;
@@ -251,7 +254,8 @@ void describe_shot(const ShotPlan &p, int variant, int chunk, int tile_begin, in
     d.nts = nts; d.ng = p.ng; d.real_nt = real_nt;
     ones_range(dom->host.prof_x, nnx, d.xin_lo, d.xin_hi);
     d.rrow_lo = rlo; d.rrow_hi = rhi; d.band_h = p.ng > 0 ? rhi - rlo + 1 : 0;
-    d.KX = dom->KX; d.KZ = dom->KZ; d.BX = dom->BX; d.BZ = dom->BZ;
+    d.KX = dom->KX; d.KZ = dom->KZ; d.BX = dom->BX; d.BZ = dom->BZ; d.B = dom->B;
+    d.dtdx = (float)dom->host.dtdx; d.dtdz = (float)dom->host.dtdz;
     d.prof_x = dom->prof_x; d.prof_z = dom->prof_z;
     d.ncell = (size_t)nnz * nnx;
 }

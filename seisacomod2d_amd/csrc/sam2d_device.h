@@ -35,6 +35,8 @@ struct ShotDev {
     const float *amp;      // [nts] source increments (fd_dt*w*kappa_s/(dx*dz))
     const float *KX, *KZ;  // dt/dx*kappa, dt/dz*kappa   (KZ may alias KX)
     const float *BX, *BZ;  // dt/dx*bu2,  dt/dz*bu1
+    const float *B;        // buoyancy 1/rho (the multi-step kernels form BX, BZ from it)
+    float dtdx, dtdz;      // fd_dt/dx, fd_dt/dz (float products, as the reference forms them)
     const float *prof_x;   // [4][nnx]: damp2a, damp2a1, damp2b, damp2b1
     const float *prof_z;   // [4][nnz]: damp1a, damp1a1, damp1b, damp1b1
     float *state;          // slots of 4 fields (px, pz, vx, vz), each ncell floats:

@@ -42,7 +42,7 @@ int upload(void *dst, const void *src, size_t bytes, hipStream_t s);
 // Coefficients of one padded domain on the device.
 struct DeviceDomain {
     DomainHost host;
-    float *KX = nullptr, *KZ = nullptr, *BX = nullptr, *BZ = nullptr;
+    float *KX = nullptr, *KZ = nullptr, *BX = nullptr, *BZ = nullptr, *B = nullptr;
     float *prof_x = nullptr, *prof_z = nullptr;
     ~DeviceDomain();
 };

@@ -43,13 +43,15 @@
 #include <type_traits>
 #include "sam2d_device.h"
 #include "sam2d_kcommon.h"
+#include "sam2d_round.h"
 
 namespace sam2d {
 
 struct ColN {
     float vx_next;              // vx(n-1) at column c+1
     float vz, px, pz, kx, kz;   // at column c (kz unused when KZ aliases KX)
-    float bx, bz, vz_j;         // BX, BZ, vz(n-1) at column c-2
+    float b;                    // buoyancy 1/rho at column c-1
+    float vz_j;                 // vz(n-1) at column c-2
     fquad s;                    // cell-interleaved state only: the state at column
                                 // c+1, kept whole until the column consumes it
 };
@@ -64,8 +66,10 @@ struct ColN {
 // per column and row brings px, pz, vx, vz of column c+1; px, pz, vz wait a
 // column (vz three) in windows, and the last step stores all four fields of
 // column jv at once, px and pz kept from its p-phase two columns earlier.
+// Three-step kernels are held to three waves per SIMD (at most 168 VGPRs),
+// the occupancy their measured best shapes need.
 template <int T, int WAVES, int PF, bool KZA, bool RING, bool AOS>
-__global__ __launch_bounds__(WAVES * 64)
+__global__ __launch_bounds__(WAVES * 64, T == 3 ? 3 : 1)
 void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_xcd, int ntiles,
              int step, int in_slot, int out_slot, int band_mask, int band_w, int4 band_s,
              int state_sample, int ngroups, int xorder)
@@ -116,8 +120,7 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
         gchar *__restrict__ so = (gchar *)slot_cells(d, out_slot);
         gcchar *__restrict__ KX = (gcchar *)d.KX;
         gcchar *__restrict__ KZ = (gcchar *)d.KZ;
-        gcchar *__restrict__ BX = (gcchar *)d.BX;
-        gcchar *__restrict__ BZ = (gcchar *)d.BZ;
+        gcchar *__restrict__ Bf = (gcchar *)d.B;
         const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);
         const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
         // Intermediate-state slots (RING only).
@@ -192,7 +195,7 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
             }
             o.kx = ld(KX, o0);
             o.kz = KZA ? 0.0f : ld(KZ, o0);
-            o.bx = ld(BX, oj); o.bz = ld(BZ, oj);
+            o.b = ld(Bf, coff(c - 1));
             return o;
         };
 
@@ -229,6 +232,11 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
             vxin[1] = ld(vxi, coff(c_first - 1));
             vxin[2] = ld(vxi, coff(c_first));
         }
+
+        // Buoyancy at the v-phase column of the first march column,
+        // jv = c_first - 2.
+        float b_0 = ld(Bf, coff(c_first - 2));
+        const float dtdx = d.dtdx, dtdz = d.dtdz;
 
         ColN q[PF];
         #pragma unroll
@@ -317,8 +325,23 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                     const float dpz = fmaf(zp0, p_d1 - pm2, zp1 * (p_d2 - p_u1));
                     float bx, bz, vx_old, vz_old;
                     if (t == 0) {
-                        bx = keep(cur.bx);
-                        bz = keep(cur.bz);
+                        // BX, BZ at jv from the buoyancy (the host's fields, bit
+                        // for bit: cpu_stagger's averages and edge rules,
+                        // cpu_modelling_kernels.cpp:64-121, times dt/dx and dt/dz
+                        // in FP64, rounded without bias)
+                        const float b_1 = keep(cur.b);            // b(jv + 1)
+                        const bool last_row = r >= nnz - 1;
+                        const float bu1 = last_row ? 0.0f : 0.5f * (lane_dn1(b_0) + b_0);
+                        float bu2;
+                        if (FAST || jv < nnx - 1) {
+                            bu2 = 0.5f * (b_1 + b_0);
+                        } else {        // last column: bu1 of column nnx - 2
+                            const float b_m1 = ld(Bf, coff(jv - 1));
+                            bu2 = last_row ? 0.0f : 0.5f * (lane_dn1(b_m1) + b_m1);
+                        }
+                        bx = round_unbiased_prod(dtdx, bu2, cell_hash(RND_BX, jv, r));
+                        bz = round_unbiased_prod(dtdz, bu1, cell_hash(RND_BZ, jv, r));
+                        b_0 = b_1;
                         vx_old = vxin[i0];
                         vz_old = AOS ? vzin[i0] : cur.vz_j;
                     } else {

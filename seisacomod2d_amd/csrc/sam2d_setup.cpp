@@ -1,5 +1,6 @@
 // sam2d_setup.cpp -- see sam2d_setup.h.
 #include "sam2d_setup.h"
+#include "sam2d_round.h"
 
 #include <cmath>
 #include <cstdint>
@@ -70,37 +71,13 @@ void coord_to_cell(const sam2d_grid &g, float x, float z, int &ix, int &iz)
     iz = (int)(z * inv_dz);
 }
 
-// FP32 value of an FP64 coefficient t by unbiased rounding: one of the two
-// floats around t, the upper with probability (t - lo) / (hi - lo), so that
-// the stored value is t on average; the draw is a hash of (field, cell), so
-// the result is the same on every run and for every kernel variant.
-// Round-to-nearest would give every cell of a homogeneous region the same
-// relative error (up to 2^-24 in each of the two coefficients whose product
-// sets the wave speed): a systematic speed error whose phase error grows
-// along the record, 2.4e-5 relative L2 after 8000 steps of the 3201 x 1201
-// salt-model window at 7.62 m against the reference's FP64 seismograms.  With
-// unbiased rounding the cells' errors average out along a wavelength, and
-// the same window comes to 1.3e-6.
-static float round_unbiased(double t, uint64_t key)
-{
-    const float n = (float)t;
-    if ((double)n == t || !std::isfinite(n)) return n;
-    const float lo = (double)n > t ? std::nextafter(n, -INFINITY) : n;
-    const float hi = std::nextafter(lo, INFINITY);
-    const double frac = (t - (double)lo) / ((double)hi - (double)lo);
-    // splitmix64 finaliser of the key: 53 uniform bits
-    uint64_t z = key + 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z ^= z >> 31;
-    const double u = (double)(z >> 11) * 0x1.0p-53;
-    return u < frac ? hi : lo;
-}
-
-static uint64_t cell_key(int field, int ix, int iz)
-{
-    return ((uint64_t)field << 56) ^ ((uint64_t)(uint32_t)ix << 28) ^ (uint64_t)(uint32_t)iz;
-}
+// The coefficient fields are rounded to FP32 without bias (sam2d_round.h):
+// with round-to-nearest, every cell of a homogeneous region would carry the
+// same relative error, a systematic wave-speed error whose phase error
+// reaches 2.4e-5 relative L2 after 8000 steps of the 3201 x 1201 salt-model
+// window at 7.62 m against the reference's FP64 seismograms; with unbiased
+// rounding the cells' errors average out along a wavelength and the same
+// window comes to 1.3e-6.
 
 void build_domain(const sam2d_grid &g, const TimeAxis &ta, const float *vp, const float *rho,
                   int wx0, int w, DomainHost &out)
@@ -148,6 +125,9 @@ void build_domain(const sam2d_grid &g, const TimeAxis &ta, const float *vp, cons
     }
     std::memcpy(&bu2[(size_t)(nnx - 1) * nnz], &bu1[(size_t)(nnx - 2) * nnz], sizeof(float) * nnz);
 
+    out.B = b;
+    out.dtdx = ta.dtdx;
+    out.dtdz = ta.dtdz;
     out.kz_alias = (ta.dtdx == ta.dtdz);
     out.KX.resize(N); out.BX.resize(N); out.BZ.resize(N);
     if (!out.kz_alias) out.KZ.resize(N); else out.KZ.clear();
@@ -159,10 +139,13 @@ void build_domain(const sam2d_grid &g, const TimeAxis &ta, const float *vp, cons
     for (int ix = 0; ix < nnx; ix++)
         for (int iz = 0; iz < nnz; iz++) {
             const size_t i = (size_t)ix * nnz + iz;
-            out.KX[i] = round_unbiased(ta.dtdx * out.kappa[i], cell_key(0, ix, iz));
-            if (!out.kz_alias) out.KZ[i] = round_unbiased(ta.dtdz * out.kappa[i], cell_key(1, ix, iz));
-            out.BX[i] = round_unbiased(ta.dtdx * (double)bu2[i], cell_key(2, ix, iz));
-            out.BZ[i] = round_unbiased(ta.dtdz * (double)bu1[i], cell_key(3, ix, iz));
+            out.KX[i] = round_unbiased(ta.dtdx * out.kappa[i], cell_hash(RND_KX, ix, iz));
+            if (!out.kz_alias)
+                out.KZ[i] = round_unbiased(ta.dtdz * out.kappa[i], cell_hash(RND_KZ, ix, iz));
+            // dtdx, dtdz are float products widened to double: dtdx * bu2 is
+            // the exact product of two floats
+            out.BX[i] = round_unbiased_prod((float)ta.dtdx, bu2[i], cell_hash(RND_BX, ix, iz));
+            out.BZ[i] = round_unbiased_prod((float)ta.dtdz, bu1[i], cell_hash(RND_BZ, ix, iz));
         }
 
     out.prof_x.assign(4 * (size_t)nnx, 0.0f);

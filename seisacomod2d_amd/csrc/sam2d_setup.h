@@ -32,8 +32,10 @@ struct DomainHost {
     int wx0, w;
     int nnx, nnz;
     bool kz_alias;                  // dtdz == dtdx: KZ is KX
+    double dtdx, dtdz;              // the time axis's fd_dt/dx, fd_dt/dz
     std::vector<double> kappa;      // FP64 bulk modulus, padded
     std::vector<float> KX, KZ, BX, BZ;
+    std::vector<float> B;           // buoyancy 1/rho (1e10 where rho <= 0), padded
     std::vector<float> prof_x;      // [4][nnx]
     std::vector<float> prof_z;      // [4][nnz]
 };
