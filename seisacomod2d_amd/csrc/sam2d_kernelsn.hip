@@ -339,8 +339,8 @@ void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_
                             const float b_m1 = ld(Bf, coff(jv - 1));
                             bu2 = last_row ? 0.0f : 0.5f * (lane_dn1(b_m1) + b_m1);
                         }
-                        bx = round_unbiased_prod(dtdx, bu2, cell_hash(RND_BX, jv, r));
-                        bz = round_unbiased_prod(dtdz, bu1, cell_hash(RND_BZ, jv, r));
+                        bx = round_unbiased_prod(dtdx, bu2, cell_draw(RND_BX, jv, r));
+                        bz = round_unbiased_prod(dtdz, bu1, cell_draw(RND_BZ, jv, r));
                         b_0 = b_1;
                         vx_old = vxin[i0];
                         vz_old = AOS ? vzin[i0] : cur.vz_j;

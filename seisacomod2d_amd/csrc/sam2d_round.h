@@ -6,8 +6,8 @@
 // A coefficient t (dt/dx * kappa, dt/dx * bu2, dt/dz * bu1, the reference's
 // FP64 products) is stored as the nearest float n or as its neighbour on
 // t's side of n, the neighbour with probability |t - n| / (its distance from
-// n), so that the stored value is t on average; the draw is a hash of
-// (field, cell), the same on every run.  Round-to-nearest would give every
+// n), so that the stored value is t on average; the draw is a fixed function
+// of (field, cell), the same on every run.  Round-to-nearest would give every
 // cell of a homogeneous region the same relative error (up to 2^-24 in each
 // of the two coefficients whose product sets the wave speed), a systematic
 // speed error whose phase error grows along the record.
@@ -21,15 +21,19 @@ namespace sam2d {
 // Fields of the draw.
 enum { RND_KX = 0, RND_KZ = 1, RND_BX = 2, RND_BZ = 3 };
 
-__host__ __device__ inline uint32_t cell_hash(uint32_t field, uint32_t ix, uint32_t iz)
+// The draw of (field, cell) as 32 fixed-point bits of [0, 1): the
+// two-dimensional Weyl sequence frac(ix * a + iz * b), with a = 1/g and
+// b = 1/g^2 for the plastic number g (g^3 = g + 1; the constants of Roberts'
+// R2 low-discrepancy sequence).  KX and KZ take (a, b), BX and BZ (b, a); KZ
+// and BZ add one half.  The cells of any region draw values spread evenly
+// over [0, 1), so the fraction rounded up follows the remainder closely, and
+// the draw costs the kernels one multiply-add per cell (the iz term is fixed
+// per lane).
+__host__ __device__ inline uint32_t cell_draw(uint32_t field, uint32_t ix, uint32_t iz)
 {
-    uint32_t h = (ix * 0x9E3779B1u) ^ ((iz + 0x7F4A7C15u) * 0x85EBCA77u) ^ (field * 0xC2B2AE3Du);
-    h ^= h >> 16;
-    h *= 0x7FEB352Du;
-    h ^= h >> 15;
-    h *= 0x846CA68Bu;
-    h ^= h >> 16;
-    return h;
+    const uint32_t a = 0xC13FA9A9u, b = 0x91E10DA6u;
+    const bool swap = field >= (uint32_t)RND_BX;
+    return ix * (swap ? b : a) + iz * (swap ? a : b) + ((field & 1u) ? 0x80000000u : 0u);
 }
 
 __host__ __device__ inline float f32_bits(uint32_t u)
