@@ -47,7 +47,9 @@ struct AdjointState {
     int cap_desc2 = 0;
     ShotDev *d_fwd = nullptr;      // forward-sweep descriptors (multi-step tile shapes)
     int cap_fwd = 0;
-    ~AdjointState() { dev_free(d_desc); dev_free(d_desc2); dev_free(d_fwd); }
+    ShotDev *d_ring = nullptr;     // recomputation descriptors, when its variant differs
+    int cap_ring = 0;
+    ~AdjointState() { dev_free(d_desc); dev_free(d_desc2); dev_free(d_fwd); dev_free(d_ring); }
 };
 
 static int grow(void **p, size_t &cap, size_t need)
@@ -290,6 +292,30 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         return SAM2D_OK;
     };
 
+    // Descriptors of the staged shots with the tile shapes of a multi-step
+    // variant and march length, in device buffer *buf (grown as needed).
+    auto multistep_desc = [&](int v, int chunk, ShotDev **buf, int *cap, int *tiles,
+                              int *groups) -> int {
+        int w2, c2;
+        variant_shape(v, &w2, &c2);
+        std::vector<ShotDev> d2(ctx->staged_desc.begin(), ctx->staged_desc.begin() + ns);
+        int t2 = 0;
+        for (auto &dd : d2) {
+            tile_shape(dd.nnz, dd.nnx, w2, chunk, variant_own_rows(v), dd.ngroups, dd.nchunks);
+            dd.chunk = chunk;
+            dd.tile_begin = t2;
+            t2 += dd.ngroups * dd.nchunks;
+        }
+        if (ns > *cap) {
+            dev_free(*buf); *buf = nullptr; *cap = 0;
+            SAM2D_TRY(dev_alloc((void **)buf, sizeof(ShotDev) * ns));
+            *cap = ns;
+        }
+        SAM2D_TRY(upload(*buf, d2.data(), sizeof(ShotDev) * ns, ctx->stream));
+        *tiles = t2;
+        *groups = d2[0].ngroups;
+        return SAM2D_OK;
+    };
     // Forward-sweep kernel: the context's variant when it advances T > 1
     // steps per launch, on descriptors with that variant's tile shapes (each
     // checkpoint segment runs its steps as T-step launches plus single steps
@@ -299,28 +325,45 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     int fwd_tiles = ntiles;
     int fwd_groups = ctx->staged_desc[0].ngroups;
     if (variant_steps(fwd_variant) > 1) {
-        int w2, c2;
-        variant_shape(fwd_variant, &w2, &c2);
-        c2 = tile_chunk(ctx, fwd_variant);
-        std::vector<ShotDev> d2(ctx->staged_desc.begin(), ctx->staged_desc.begin() + ns);
-        int t2 = 0;
-        for (auto &dd : d2) {
-            tile_shape(dd.nnz, dd.nnx, w2, c2, variant_own_rows(fwd_variant), dd.ngroups, dd.nchunks);
-            dd.chunk = c2;
-            dd.tile_begin = t2;
-            t2 += dd.ngroups * dd.nchunks;
-        }
-        if (ns > A.cap_fwd) {
-            dev_free(A.d_fwd); A.d_fwd = nullptr; A.cap_fwd = 0;
-            SAM2D_TRY(dev_alloc((void **)&A.d_fwd, sizeof(ShotDev) * ns));
-            A.cap_fwd = ns;
-        }
-        SAM2D_TRY(upload(A.d_fwd, d2.data(), sizeof(ShotDev) * ns, ctx->stream));
+        SAM2D_TRY(multistep_desc(fwd_variant, tile_chunk(ctx, fwd_variant), &A.d_fwd, &A.cap_fwd,
+                                 &fwd_tiles, &fwd_groups));
         fwd_desc = A.d_fwd;
-        fwd_tiles = t2;
-        fwd_groups = d2[0].ngroups;
     } else {
         fwd_variant = variant;
+    }
+    // Recomputation kernel: the ring form of a three-step variant, which
+    // stores every state it passes.  The sweep's variant when it is one
+    // (with its tuned march length and tile order); else the three-step
+    // variant the autotuner timed fastest (at its own march length, plain
+    // order), else 1 wave x 128 columns with prefetch 4.  Every variant gives
+    // the same bits.
+    int ring_variant = fwd_variant;
+    const ShotDev *ring_desc = fwd_desc;
+    int ring_tiles = fwd_tiles, ring_groups = fwd_groups, ring_order = ctx->tile_order;
+    if (!ring_supported(fwd_variant)) {
+        ring_variant = -1;
+        for (int v = 0; v < (int)ctx->tune_ms.size(); v++)
+            if (ring_supported(v) && ctx->tune_ms[v] > 0.0 &&
+                (ring_variant < 0 || ctx->tune_ms[v] < ctx->tune_ms[ring_variant]))
+                ring_variant = v;
+        if (ring_variant < 0) {
+            for (int v = 0; v < num_variants(); v++) {
+                int w, c, pf;
+                variant_shape_pf(v, &w, &c, &pf);
+                if (ring_supported(v) && w == 1 && c == 128 && pf == 4) ring_variant = v;
+            }
+        }
+        // one array per field in the ring: 32-bit byte offsets up to 2^30 cells
+        for (int s = 0; s < ns; s++)
+            if (ctx->bufs[s]->cells >= ((size_t)1 << 30)) ring_variant = -1;
+        if (ring_variant >= 0) {
+            int w, c;
+            variant_shape(ring_variant, &w, &c);
+            SAM2D_TRY(multistep_desc(ring_variant, march_aligned(c, 15), &A.d_ring, &A.cap_ring,
+                                     &ring_tiles, &ring_groups));
+            ring_desc = A.d_ring;
+            ring_order = 0;
+        }
     }
 
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
@@ -346,10 +389,10 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         // steps per launch with the ring form of a three-step variant, single
         // steps for the rest.
         int n = n0;
-        if (ring_supported(fwd_variant))
+        if (ring_variant >= 0)
             for (; n + 2 < n1; n += 3)
-                HIP_TRY(launch_stepn_ring(fwd_variant, fwd_desc, ns, fwd_tiles, ctx->grid_cap, kza,
-                                          n, n - n0, TileOrder{fwd_groups, ctx->tile_order},
+                HIP_TRY(launch_stepn_ring(ring_variant, ring_desc, ns, ring_tiles, ctx->grid_cap,
+                                          kza, n, n - n0, TileOrder{ring_groups, ring_order},
                                           ctx->stream));
         // Ring slots 1, 2, 4, 5, ... below the last three-step launch hold p
         // in place of px and pz.

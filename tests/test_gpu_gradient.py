@@ -107,7 +107,8 @@ def test_multistep_forward_sweep_bitwise():
         m.set_option("variant", names.index("waves2_chunk32_pf2"))
         ref = m.gradient(shots, adj, ckpt_k=10, want_seis=True)
         for name in ("twostep_waves2_chunk32", "twostep_waves4_chunk64_pf4",
-                     "threestep_waves1_chunk32", "fourstep_waves2_chunk64"):
+                     "threestep_waves1_chunk32", "fourstep_waves2_chunk64",
+                     "fivestep_waves1_chunk32"):
             m.set_option("variant", names.index(name))
             for k in (12, 10, 7, 3):          # not a multiple: single-step sweep
                 g = m.gradient(shots, adj, ckpt_k=k, want_seis=True)
