@@ -322,7 +322,7 @@ def bench_batched(args, sam, nshots=64, t_rec=4.0, batch_sizes=(0, 8, 16)):
 
 # --------------------------------------------------------------- configs 3, 4
 def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False, adj_steps=2,
-               fixed=None):
+               fixed=None, budget_gb=0):
     """Gradient time against the forward modelling of the same shots: the
     batched forward (autotuned, the fastest forward path) and the per-shot
     loop.  fixed: (variant, chunk, tile order) to use instead of the tuner
@@ -343,6 +343,8 @@ def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False, a
         if no_graph:
             m.set_option("graph", 0)
         m.set_option("adjoint_steps", adj_steps)
+        if budget_gb > 0:
+            m.set_option("hbm_budget_gb", budget_gb)
         # per-shot loop with a variant tuned on one shot; one untimed batched
         # pass sets up every window's coefficients first, as the timed
         # batched and gradient calls find them
@@ -392,7 +394,7 @@ def gradient_c4(args, sam, nshots=32, ks=(100,)):
     sh4, w4 = salt_shots(sam, nshots, seed=11)
     return _grad_case(sam, grid4, vs, v4, rs, sh4, w4, list(ks),
                       "config4_gradient_3201x1201_nt8000_windows", args.no_graph, args.adjoint_steps,
-                      _fixed(args, sam))
+                      _fixed(args, sam), args.hbm_budget_gb)
 
 
 def bench_gradient(args, sam):
@@ -405,7 +407,7 @@ def bench_gradient(args, sam):
     if not args.c4_only:
         out += _grad_case(sam, grid, vtrue, v0, rho, sh, None, [100],
                           "config3_gradient_1201x601_nt3000", args.no_graph, args.adjoint_steps,
-                          _fixed(args, sam))
+                          _fixed(args, sam), args.hbm_budget_gb)
     ks = [int(k) for k in args.ckpt_k.split(",")] if args.ckpt_k else (
         [200] if args.quick else [50, 100, 200])
     out += gradient_c4(args, sam, args.shots or (4 if args.quick else 32), ks)
@@ -432,6 +434,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2, choices=(1, 2),
                     help="gradient workload: backward steps per adjoint pass")
+    ap.add_argument("--hbm-budget-gb", type=int, default=0,
+                    help="gradient workload: device memory a gradient batch may use "
+                         "(0: the library's default)")
     ap.add_argument("--workload", default="plain", choices=["plain", "batched", "gradient"])
     ap.add_argument("--quick", action="store_true", help="batched/gradient: fewer shots")
     ap.add_argument("--max-batch", type=int, default=0)

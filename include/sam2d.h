@@ -179,8 +179,10 @@ int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
  * shot after the option is set, and the first after any other modelling call,
  * starts fresh; default 0 = fresh state for every shot),
  * "max_batch" (shots resident at once in batched and gradient calls, 0 = all
- * or, for gradients, as many as fit "hbm_budget_gb" (default 64 GiB of device
- * memory for ring, checkpoints and adjoint state)), "adjoint_steps" (backward
+ * or, for gradients, the fewest batches of equal shot counts that fit
+ * "hbm_budget_gb" GiB of device memory for ring, checkpoints and adjoint
+ * state (default 0 = the device memory that is free or held by the context's
+ * reusable buffers, less a reserve of 5 % of the device, at most 90 %)), "adjoint_steps" (backward
  * steps per adjoint kernel pass in sam2d_gradient, 1 or 2, default 2; both
  * give the same bits), "adjoint_variant" (tile shape of the adjoint kernels,
  * the single-step form of the given variant index; -1 = automatic, the

@@ -487,18 +487,44 @@ extern "C" int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshot
         seis_ptr[s] = seis ? seis + off : nullptr;
         off += (size_t)shots[s].nrec * ta.real_nt;
     }
-    // Shots per batch: the option, else as many as fit the HBM budget.
+    // Shots per batch: the option, else the fewest batches of equal shot
+    // counts whose ring, checkpoints and adjoint state fit the HBM budget:
+    // "hbm_budget_gb", or by default the device memory that is free or held
+    // by this context's reusable buffers, less a reserve of 5 % of the device
+    // (at least 4 GiB), and at most 90 % of the device.  The 288 GB of an
+    // MI355X hold a 32-shot C4 batch at k = 100 (217 GB) in one piece.
     int batch = ctx->max_batch > 0 ? ctx->max_batch : nshots;
     if (ctx->max_batch <= 0 && nshots > 0) {
-        const size_t budget = (size_t)ctx->hbm_budget_gb << 30;
-        size_t used = 0;
-        int fit = 0;
-        for (int s = 0; s < nshots; s++) {
-            used += shot_bytes(plans[s], ta.fd_nt, k);
-            if (used > budget && fit > 0) break;
-            fit++;
+        size_t budget = (size_t)ctx->hbm_budget_gb << 30;
+        if (ctx->hbm_budget_gb <= 0) {
+            size_t free_b = 0, total_b = 0;
+            HIP_TRY(hipSetDevice(ctx->device));
+            HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+            size_t held = 0;
+            for (auto &b : ctx->bufs)
+                if (b) held += b->cap_state * sizeof(float);
+            for (auto &b : ctx->adj->bufs)
+                if (b)
+                    for (size_t c : b->cap) held += c;
+            const size_t reserve = std::max<size_t>((size_t)4 << 30, total_b / 20);
+            const size_t avail = free_b + held > reserve ? free_b + held - reserve : 0;
+            budget = std::min(avail, total_b / 10 * 9);
         }
-        batch = std::max(1, fit);
+        auto fits = [&](int b) {
+            for (int s0 = 0; s0 < nshots; s0 += b) {
+                size_t used = 0;
+                for (int s = s0; s < std::min(nshots, s0 + b); s++)
+                    used += shot_bytes(plans[s], ta.fd_nt, k);
+                if (used > budget && b > 1) return false;
+            }
+            return true;
+        };
+        int nb = 1;
+        batch = nshots;
+        while (batch > 1 && !fits(batch)) {
+            nb++;
+            batch = (nshots + nb - 1) / nb;
+        }
     }
     std::vector<double> gv((size_t)ctx->grid.nx * ctx->grid.nz, 0.0), gr(gv.size(), 0.0);
     // The adjoint reads single fields of the forward states: the

@@ -749,7 +749,7 @@ int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value)
     else if (k == "max_batch") ctx->max_batch = value < 0 ? 0 : value;
     else if (k == "grid_cap") ctx->grid_cap = value < 0 ? 0 : value;
     else if (k == "hbm_budget_gb") {
-        if (value < 1) return fail(SAM2D_ERR_PARAM, "hbm_budget_gb must be >= 1");
+        if (value < 0) return fail(SAM2D_ERR_PARAM, "hbm_budget_gb must be >= 0");
         ctx->hbm_budget_gb = value;
     } else if (k == "adjoint_steps") {
         if (value != 1 && value != 2) return fail(SAM2D_ERR_PARAM, "adjoint_steps must be 1 or 2");

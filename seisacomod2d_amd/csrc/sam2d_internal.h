@@ -114,7 +114,7 @@ struct sam2d_ctx {
     int variant = -1, tuned_variant = -1, default_variant = 0;
     int max_batch = 0;
     int grid_cap = 0;
-    int hbm_budget_gb = 64;                // device memory a gradient batch may use
+    int hbm_budget_gb = 0;                 // device memory a gradient batch may use (0: automatic)
     int adjoint_steps = 2;                 // backward steps per adjoint launch (1 or 2)
     int adjoint_variant = -1;              // single-step variant for adjoint tiles, -1 = auto
     int tile_order = 0;                    // chunk-column stride of multi-step kernels (tuned)
