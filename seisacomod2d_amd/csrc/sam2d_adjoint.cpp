@@ -332,7 +332,8 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         fwd_variant = variant;
     }
     // Recomputation kernel: the ring form of a three-step variant, which
-    // stores every state it passes.  The sweep's variant when it is one
+    // stores every state it passes.  The "ring_variant" option (march length
+    // "ring_chunk"); else the sweep's variant when it is a three-step one
     // (with its tuned march length and tile order); else the three-step
     // variant the autotuner timed fastest (at its own march length, plain
     // order), else 1 wave x 128 columns with prefetch 4.  Every variant gives
@@ -340,9 +341,9 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     int ring_variant = fwd_variant;
     const ShotDev *ring_desc = fwd_desc;
     int ring_tiles = fwd_tiles, ring_groups = fwd_groups, ring_order = ctx->tile_order;
-    if (!ring_supported(fwd_variant)) {
-        ring_variant = -1;
-        for (int v = 0; v < (int)ctx->tune_ms.size(); v++)
+    if (ctx->ring_variant >= 0 || !ring_supported(fwd_variant)) {
+        ring_variant = ctx->ring_variant;             // the option, else
+        for (int v = 0; ctx->ring_variant < 0 && v < (int)ctx->tune_ms.size(); v++)
             if (ring_supported(v) && ctx->tune_ms[v] > 0.0 &&
                 (ring_variant < 0 || ctx->tune_ms[v] < ctx->tune_ms[ring_variant]))
                 ring_variant = v;
@@ -359,8 +360,9 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         if (ring_variant >= 0) {
             int w, c;
             variant_shape(ring_variant, &w, &c);
-            SAM2D_TRY(multistep_desc(ring_variant, march_aligned(c, 15), &A.d_ring, &A.cap_ring,
-                                     &ring_tiles, &ring_groups));
+            const int rc = ctx->ring_chunk > 0 ? ctx->ring_chunk : march_aligned(c, 15);
+            SAM2D_TRY(multistep_desc(ring_variant, rc, &A.d_ring, &A.cap_ring, &ring_tiles,
+                                     &ring_groups));
             ring_desc = A.d_ring;
             ring_order = 0;
         }

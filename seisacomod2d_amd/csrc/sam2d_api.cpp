@@ -767,6 +767,14 @@ int sam2d_set_option(sam2d_ctx *ctx, const char *key, int value)
     } else if (k == "adjoint_variant") {
         if (value >= num_variants()) return fail(SAM2D_ERR_PARAM, "variant %d out of range", value);
         ctx->adjoint_variant = value < 0 ? -1 : variant_single_step(value);
+    } else if (k == "ring_variant") {
+        if (value >= num_variants() || (value >= 0 && !ring_supported(value)))
+            return fail(SAM2D_ERR_PARAM, "ring_variant %d is not a three-step variant", value);
+        ctx->ring_variant = value < 0 ? -1 : value;
+    } else if (k == "ring_chunk") {
+        if (value != 0 && (value < 8 || value > 4096))
+            return fail(SAM2D_ERR_PARAM, "ring_chunk must be 0 or 8..4096");
+        ctx->ring_chunk = value;
     }
     else return fail(SAM2D_ERR_PARAM, "unknown option '%s'", key);
     return SAM2D_OK;
@@ -785,6 +793,8 @@ int sam2d_get_option(sam2d_ctx *ctx, const char *key, int *value)
     else if (k == "hbm_budget_gb") *value = ctx->hbm_budget_gb;
     else if (k == "adjoint_steps") *value = ctx->adjoint_steps;
     else if (k == "adjoint_variant") *value = ctx->adjoint_variant;
+    else if (k == "ring_variant") *value = ctx->ring_variant;
+    else if (k == "ring_chunk") *value = ctx->ring_chunk;
     else if (k == "tile_order") *value = ctx->tile_order;
     else if (k == "adjoint_tile_order") *value = ctx->adjoint_tile_order;
     else if (k == "chunk") *value = ctx->chunk;

@@ -117,6 +117,8 @@ struct sam2d_ctx {
     int hbm_budget_gb = 0;                 // device memory a gradient batch may use (0: automatic)
     int adjoint_steps = 2;                 // backward steps per adjoint launch (1 or 2)
     int adjoint_variant = -1;              // single-step variant for adjoint tiles, -1 = auto
+    int ring_variant = -1;                 // three-step variant of the recomputation, -1 = auto
+    int ring_chunk = 0;                    // its march length, 0 = the variant's
     int tile_order = 0;                    // chunk-column stride of multi-step kernels (tuned)
     int adjoint_tile_order = 7;            // chunk-column stride of the adjoint kernels
     int chunk = 0;                         // march length of multi-step kernels, 0 = variant's
