@@ -390,7 +390,8 @@ def gradient_c4(args, sam, nshots=32, ks=(100,)):
     from scipy.ndimage import gaussian_filter
     vs, rs = synthetic.salt_model()
     v4 = gaussian_filter(vs.astype(np.float64), 20.0).astype(np.float32)
-    grid4 = sam.Grid(3201, 1201, 7.62, 7.62, 4.0, 0.0005, 0.0005, 15.0, npml=60)
+    t_rec = 4.0 if args.profile_steps <= 0 else args.profile_steps * 0.0005
+    grid4 = sam.Grid(3201, 1201, 7.62, 7.62, t_rec, 0.0005, 0.0005, 15.0, npml=60)
     sh4, w4 = salt_shots(sam, nshots, seed=11)
     return _grad_case(sam, grid4, vs, v4, rs, sh4, w4, list(ks),
                       "config4_gradient_3201x1201_nt8000_windows", args.no_graph, args.adjoint_steps,
@@ -401,7 +402,8 @@ def bench_gradient(args, sam):
     out = []
     vtrue, v0, rho = synthetic.anomaly_model()
     n3 = 4 if args.quick else 16
-    grid = sam.Grid(1201, 601, 10.0, 10.0, 3.0, 0.001, 0.001, 10.0, npml=32)
+    t3 = 3.0 if args.profile_steps <= 0 else args.profile_steps * 0.001
+    grid = sam.Grid(1201, 601, 10.0, 10.0, t3, 0.001, 0.001, 10.0, npml=32)
     rec = np.stack([np.arange(1201) * 10.0, np.full(1201, 10.0)], 1).astype(np.float32)
     sh = [sam.Shot(10.0 * int((k + 0.5) * 1201 / n3), 10.0, rec) for k in range(n3)]
     if not args.c4_only:
@@ -444,7 +446,8 @@ def main():
     ap.add_argument("--ckpt-k", default="", help="gradient workload: checkpoint intervals, e.g. 100")
     ap.add_argument("--shots", type=int, default=0, help="gradient workload: salt-model shots")
     ap.add_argument("--profile-steps", type=int, default=0,
-                    help="profiling runs only: shorten each shot to this many time steps")
+                    help="profiling runs only: shorten each shot (each gradient record) to "
+                         "this many time steps")
     ap.add_argument("--tuning", action="store_true", help="print per-variant autotune times")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -25,8 +25,12 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
-LIB_SOURCES = ["sam2d_kernels.hip", "sam2d_kernelsn.hip", "sam2d_adjoint_kernels.hip", "sam2d_api.cpp",
-               "sam2d_adjoint.cpp", "sam2d_setup.cpp", "sam2d_noalloc.hip"]
+# The multi-step forward kernel is instantiated per depth in translation units
+# of its own (sam2d_kernelsn_t*.hip), which compile in parallel.
+LIB_SOURCES = ["sam2d_kernelsn_t3.hip", "sam2d_kernelsn_t3_ring.hip", "sam2d_kernelsn_t4.hip",
+               "sam2d_kernelsn_t5.hip", "sam2d_kernelsn_t2.hip", "sam2d_adjoint_kernels.hip",
+               "sam2d_kernels.hip", "sam2d_kernelsn.hip", "sam2d_noalloc.hip", "sam2d_api.cpp",
+               "sam2d_adjoint.cpp", "sam2d_setup.cpp"]
 HOST_SOURCES = ["jobcard.cpp", "geometry.cpp", "model_io.cpp", "modelling_module.cpp"]
 
 

@@ -4,12 +4,13 @@
 //   1. forward over all fd_nt steps in two ring slots, saving the state that
 //      enters step m*k into checkpoint m (and the seismograms if asked for);
 //   2. backward, segment by segment from the last: restore checkpoint m into
-//      ring slot 0, recompute steps [m*k, m*k + k) into ring slots 1..k
-//      (bit-identical to the first pass: the same arithmetic per cell, three
-//      steps per launch where the forward variant allows), then run the
-//      adjoint steps of the segment in reverse, two per pass where every shot
-//      has a residual band, each reading the forward states it needs from the
-//      ring;
+//      state slot 0, recompute steps [m*k, m*k + k) (bit-identical to the
+//      first pass: the same arithmetic per cell, three steps per launch where
+//      a three-step variant fits), storing for each step n the triplet
+//      (p after n, vx and vz before n) of every cell into ring slot n - m*k,
+//      then run the adjoint steps of the segment in reverse, two per pass
+//      where every shot has a residual band, step n reading ring slot
+//      n - m*k;
 //   3. copy dJ/dkappa, dJ/dbu2, dJ/dbu1 to the host and chain them in FP64 to
 //      dJ/dvp and dJ/drho through the buoyancy staggering, kappa = vp^2 rho and
 //      the sponge padding (the transpose of cpu_stagger,
@@ -23,17 +24,19 @@
 namespace sam2d {
 
 struct AdjShotBuffers {
-    float *lam = nullptr;     // 2 adjoint slots x 4 fields, then gk, gbx, gbz
-    float *ckpt = nullptr;    // nckpt slots x 4 fields
+    float *lam = nullptr;     // 2 adjoint slots x 4 fields, then the accumulator cells
+                              // (dJ/dkappa, dJ/dbu2, dJ/dbu1), all cell-interleaved
+    float *ckpt = nullptr;    // nckpt state slots (cell-interleaved)
+    float *ring = nullptr;    // k triplet slots (ShotDev::ring)
     float *srcg = nullptr;
     int *rcell = nullptr, *rptr = nullptr, *rid = nullptr;
     float *res = nullptr;
     float *rband = nullptr;
     int *bmap = nullptr;
-    size_t cap[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // bytes of each buffer above
+    size_t cap[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // bytes of each buffer above
     ~AdjShotBuffers()
     {
-        dev_free(lam); dev_free(ckpt); dev_free(srcg);
+        dev_free(lam); dev_free(ckpt); dev_free(ring); dev_free(srcg);
         dev_free(rcell); dev_free(rptr); dev_free(rid); dev_free(res); dev_free(rband);
         dev_free(bmap);
     }
@@ -47,7 +50,7 @@ struct AdjointState {
     int cap_desc2 = 0;
     ShotDev *d_fwd = nullptr;      // forward-sweep descriptors (multi-step tile shapes)
     int cap_fwd = 0;
-    ShotDev *d_ring = nullptr;     // recomputation descriptors, when its variant differs
+    ShotDev *d_ring = nullptr;     // recomputation descriptors (three-step tile shapes)
     int cap_ring = 0;
     ~AdjointState() { dev_free(d_desc); dev_free(d_desc2); dev_free(d_fwd); dev_free(d_ring); }
 };
@@ -113,12 +116,28 @@ static void chain_to_model(const sam2d_ctx *ctx, const DomainHost &h, const std:
     }
 }
 
-// Device bytes one shot of `plan` needs for a gradient with checkpoint interval k.
+// Device bytes one shot of `plan` needs for a gradient with checkpoint
+// interval k: two state slots, k ring triplets, the checkpoints, the adjoint
+// state with its accumulators and the adjoint's coefficient cells.
 static size_t shot_bytes(const ShotPlan &p, int fd_nt, int k)
 {
     const size_t N = (size_t)p.dom->host.nnx * p.dom->host.nnz;
     const size_t nck = (size_t)(fd_nt + k - 1) / k;
-    return N * 4 * (size_t)(k + 1 + nck + 2) * sizeof(float) + 3 * N * sizeof(float);
+    return N * sizeof(float) * (2 * 4 + 3 * (size_t)k + 4 * nck + 11 + 4);
+}
+
+// The three-step variant with the tile shape of v and two columns of operands
+// in flight: the recomputation form needs no spills at that depth.
+static int ring_form(int v)
+{
+    int w, c, pf;
+    variant_shape_pf(v, &w, &c, &pf);
+    for (int u = 0; u < num_variants(); u++) {
+        int w2, c2, pf2;
+        variant_shape_pf(u, &w2, &c2, &pf2);
+        if (variant_steps(u) == 3 && w2 == w && c2 == c && pf2 == 2) return u;
+    }
+    return v;
 }
 
 static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const float *const *adj,
@@ -137,7 +156,8 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
                         : ctx->adjoint_steps == 2 ? variant_index(4, 32, 2)
                                                   : variant_index(1, 64, 2);
     int ntiles = 0;
-    SAM2D_TRY(stage_batch(ctx, plans, variant, &ntiles, k + 1));
+    SAM2D_TRY(stage_batch(ctx, plans, variant, &ntiles, 2));
+    const bool kza = ta.dtdx == ta.dtdz;
     if ((int)A.bufs.size() < ns) A.bufs.resize(ns);
     std::vector<AdjDev> desc(ns);
     std::vector<int> ncells(ns, 0);
@@ -156,12 +176,20 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         const size_t ng1 = (size_t)std::max(1, p.ng);
         SAM2D_TRY(grow((void **)&b.lam, b.cap[0], 11 * N * sizeof(float)));
         SAM2D_TRY(grow((void **)&b.ckpt, b.cap[1], (size_t)nseg * 4 * N * sizeof(float)));
+        SAM2D_TRY(grow((void **)&b.ring, b.cap[9], (size_t)k * 3 * N * sizeof(float)));
+        ctx->staged_desc[s].ring = b.ring;
+        // The adjoint's coefficient cells of this domain, built once.
+        if (!p.dom->ADJ) {
+            SAM2D_TRY(dev_alloc((void **)&p.dom->ADJ, (kza ? 3 : 4) * N * sizeof(float)));
+            HIP_TRY(launch_pack_coef(p.dom->KX, kza ? nullptr : p.dom->KZ, p.dom->BX, p.dom->BZ, N,
+                                     p.dom->ADJ, ctx->stream));
+        }
         SAM2D_TRY(grow((void **)&b.srcg, b.cap[2], (size_t)std::max(1, ta.nts) * sizeof(float)));
         SAM2D_TRY(grow((void **)&b.rcell, b.cap[3], ng1 * sizeof(int)));
         SAM2D_TRY(grow((void **)&b.rptr, b.cap[4], (ng1 + 1) * sizeof(int)));
         SAM2D_TRY(grow((void **)&b.rid, b.cap[5], ng1 * sizeof(int)));
         SAM2D_TRY(grow((void **)&b.res, b.cap[6], ng1 * ta.real_nt * sizeof(float)));
-        float *lam = b.lam, *grad = b.lam + 8 * N;
+        float *lam = b.lam, *acc = b.lam + 8 * N;
         HIP_TRY(launch_fill(lam, 11 * N, 0.0f, ctx->stream));
 
         // Source-weight per step and receivers by column (CSR).
@@ -228,10 +256,10 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         d.src_x = p.src_x; d.src_z = p.src_z; d.nts = ta.nts; d.real_nt = ta.real_nt;
         d.dtdx = (float)ta.dtdx; d.dtdz = (float)ta.dtdz;
         d.srcg = b.srcg;
-        d.KX = fd.KX; d.KZ = fd.KZ; d.BX = fd.BX; d.BZ = fd.BZ;
+        d.coef = p.dom->ADJ;
         d.prof_x = fd.prof_x; d.prof_z = fd.prof_z;
-        d.fwd = fd.state; d.lam = lam; d.ncell = N;
-        d.gk = grad; d.gbx = grad + N; d.gbz = grad + 2 * N;
+        d.ring = b.ring; d.lam = lam; d.ncell = N;
+        d.acc = acc;
         d.xin_lo = fd.xin_lo; d.xin_hi = fd.xin_hi;
         d.ncells = ncells[s];
         d.rcell = b.rcell; d.rptr = b.rptr; d.rid = b.rid; d.res = b.res;
@@ -244,6 +272,8 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         A.cap_desc = ns;
     }
     SAM2D_TRY(upload(A.d_desc, desc.data(), sizeof(AdjDev) * ns, ctx->stream));
+    // The staged descriptors now carry the rings (single-step recomputation).
+    SAM2D_TRY(upload(ctx->d_desc, ctx->staged_desc.data(), sizeof(ShotDev) * ns, ctx->stream));
     // Two backward steps per pass need every residual in a band (nothing to
     // inject between the steps) and strips of OWN_Z2 rows.
     const bool two_step = ctx->adjoint_steps == 2 && all_band;
@@ -270,24 +300,16 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     int max_ng = 0, max_cells = 0;
     for (auto &p : plans) max_ng = std::max(max_ng, p.ng);
     for (int c : ncells) max_cells = std::max(max_cells, c);
-    const bool kza = ta.dtdx == ta.dtdz;
 
-    // The forward sweep runs on cell-interleaved state where the multi-step
-    // kernels' 32-bit offsets allow it (16 bytes per cell).  Checkpoints are
-    // copies of its state; a restore rearranges one into ring slot 0 as one
-    // array per field.
-    bool sweep_aos = true;
-    for (int s = 0; s < ns; s++) sweep_aos = sweep_aos && ctx->bufs[s]->cells < ((size_t)1 << 28);
+    // Checkpoint m is a copy of the state entering step m*k; a restore copies
+    // it back into state slot 0.
     auto save_or_restore = [&](int m, int slot, bool save) -> int {
         for (int s = 0; s < ns; s++) {
             ShotBuffers &fb = *ctx->bufs[s];
             float *ck = A.bufs[s]->ckpt + (size_t)m * 4 * fb.cells;
-            if (save || !sweep_aos)
-                HIP_TRY(hipMemcpyAsync(save ? ck : fb.slot(slot), save ? fb.slot(slot) : ck,
-                                       4 * fb.cells * sizeof(float), hipMemcpyDeviceToDevice,
-                                       ctx->stream));
-            else
-                HIP_TRY(launch_cells_to_fields(ck, fb.slot(slot), fb.cells, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(save ? ck : fb.slot(slot), save ? fb.slot(slot) : ck,
+                                   4 * fb.cells * sizeof(float), hipMemcpyDeviceToDevice,
+                                   ctx->stream));
         }
         return SAM2D_OK;
     };
@@ -331,41 +353,38 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     } else {
         fwd_variant = variant;
     }
-    // Recomputation kernel: the ring form of a three-step variant, which
-    // stores every state it passes.  The "ring_variant" option (march length
-    // "ring_chunk"); else the sweep's variant when it is a three-step one
-    // (with its tuned march length and tile order); else the three-step
-    // variant the autotuner timed fastest (at its own march length, plain
-    // order), else 1 wave x 128 columns with prefetch 4.  Every variant gives
-    // the same bits.
+    // Recomputation kernel: the ring form of a three-step variant.  The
+    // "ring_variant" option (march length "ring_chunk"); else the sweep's
+    // variant when it is a three-step one (with its tuned march length and
+    // tile order); else the three-step variant the autotuner timed fastest
+    // (at its own march length, plain order), else 1 wave x 128 columns.
+    // Without the option, at prefetch depth 2.  Every variant gives the
+    // same bits.
     int ring_variant = fwd_variant;
     const ShotDev *ring_desc = fwd_desc;
     int ring_tiles = fwd_tiles, ring_groups = fwd_groups, ring_order = ctx->tile_order;
-    if (ctx->ring_variant >= 0 || !ring_supported(fwd_variant)) {
+    int ring_chunk = tile_chunk(ctx, fwd_variant);
+    bool own_desc = ctx->ring_variant >= 0 || !ring_supported(fwd_variant);
+    if (own_desc) {
         ring_variant = ctx->ring_variant;             // the option, else
         for (int v = 0; ctx->ring_variant < 0 && v < (int)ctx->tune_ms.size(); v++)
             if (ring_supported(v) && ctx->tune_ms[v] > 0.0 &&
                 (ring_variant < 0 || ctx->tune_ms[v] < ctx->tune_ms[ring_variant]))
                 ring_variant = v;
-        if (ring_variant < 0) {
-            for (int v = 0; v < num_variants(); v++) {
-                int w, c, pf;
-                variant_shape_pf(v, &w, &c, &pf);
-                if (ring_supported(v) && w == 1 && c == 128 && pf == 4) ring_variant = v;
-            }
-        }
-        // one array per field in the ring: 32-bit byte offsets up to 2^30 cells
-        for (int s = 0; s < ns; s++)
-            if (ctx->bufs[s]->cells >= ((size_t)1 << 30)) ring_variant = -1;
-        if (ring_variant >= 0) {
-            int w, c;
-            variant_shape(ring_variant, &w, &c);
-            const int rc = ctx->ring_chunk > 0 ? ctx->ring_chunk : march_aligned(c, 15);
-            SAM2D_TRY(multistep_desc(ring_variant, rc, &A.d_ring, &A.cap_ring, &ring_tiles,
-                                     &ring_groups));
-            ring_desc = A.d_ring;
-            ring_order = 0;
-        }
+        if (ring_variant < 0) ring_variant = ring_form(variant_index(1, 128, 4));
+        int w, c;
+        variant_shape(ring_variant, &w, &c);
+        ring_chunk = ctx->ring_chunk > 0 ? ctx->ring_chunk : march_aligned(c, 15);
+        ring_order = 0;
+    }
+    if (ctx->ring_variant < 0 && ring_form(ring_variant) != ring_variant) {
+        ring_variant = ring_form(ring_variant);
+        own_desc = true;
+    }
+    if (own_desc) {
+        SAM2D_TRY(multistep_desc(ring_variant, ring_chunk, &A.d_ring, &A.cap_ring, &ring_tiles,
+                                 &ring_groups));
+        ring_desc = A.d_ring;
     }
 
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
@@ -374,42 +393,34 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     //    (cell-interleaved state, as the forward-modelling calls)
     const bool want_seis = seis_out != nullptr;
     int cur = 0;
-    ctx->state_aos = sweep_aos;
     for (int m = 0; m < nseg; m++) {
         SAM2D_TRY(save_or_restore(m, cur, true));
         const int n0 = m * k, n1 = std::min(ta.fd_nt, n0 + k);
         SAM2D_TRY(enqueue_steps_on(ctx, fwd_desc, fwd_groups, ns, fwd_tiles, fwd_variant, n0, n1,
                                    want_seis ? max_ng : 0, cur, &cur));
     }
-    ctx->state_aos = false;
     // 2. backward by segments
     int lam = 0;
     for (int m = nseg - 1; m >= 0; m--) {
         const int n0 = m * k, n1 = std::min(ta.fd_nt, n0 + k);
         SAM2D_TRY(save_or_restore(m, 0, false));
-        // Recompute the segment's states into ring slots 1 .. n1-n0: three
-        // steps per launch with the ring form of a three-step variant, single
-        // steps for the rest.
-        int n = n0;
-        if (ring_variant >= 0)
-            for (; n + 2 < n1; n += 3)
+        // Recompute the segment, step n's triplets into ring slot n - n0:
+        // three steps per launch, single steps for the rest.
+        int n = n0, st = 0;
+        if (ring_supported(ring_variant))
+            for (; n + 2 < n1; n += 3, st ^= 1)
                 HIP_TRY(launch_stepn_ring(ring_variant, ring_desc, ns, ring_tiles, ctx->grid_cap,
-                                          kza, n, n - n0, TileOrder{ring_groups, ring_order},
+                                          kza, n, st, n - n0, TileOrder{ring_groups, ring_order},
                                           ctx->stream));
-        // Ring slots 1, 2, 4, 5, ... below the last three-step launch hold p
-        // in place of px and pz.
-        const int ring_end = n - n0;
-        auto psum = [&](int slot) -> int { return slot <= ring_end && slot % 3 != 0 ? 1 : 0; };
-        for (; n < n1; n++)
-            HIP_TRY(launch_step(variant, ctx->d_desc, ns, ntiles, ctx->grid_cap, n, n - n0,
-                                n - n0 + 1, -1, false, ctx->stream));
+        for (; n < n1; n++, st ^= 1)
+            HIP_TRY(launch_step_ring(variant, ctx->d_desc, ns, ntiles, ctx->grid_cap, n, st, st ^ 1,
+                                     n - n0, ctx->stream));
         int n_top = n1 - 1;
         if (two_step)
             for (; n_top - 1 >= n0; n_top -= 2) {
-                const int fc = n_top - n0 + 1, fb = n_top - n0;
                 HIP_TRY(launch_adjoint_step2(variant, A.d_desc2, ns, ntiles2, kza, n_top, lam,
-                                             lam ^ 1, fb - 1, fb, fc, rec_sample(ta, n_top),
-                                             rec_sample(ta, n_top - 1), psum(fc) | psum(fb) << 1,
+                                             lam ^ 1, n_top - n0, rec_sample(ta, n_top),
+                                             rec_sample(ta, n_top - 1),
                                              TileOrder{groups2, ctx->adjoint_tile_order},
                                              ctx->stream));
                 lam ^= 1;
@@ -418,7 +429,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
             if (!all_band)
                 HIP_TRY(launch_inject(A.d_desc, ns, max_cells, lam, rec_sample(ta, n), ctx->stream));
             HIP_TRY(launch_adjoint_step(variant, A.d_desc, ns, ntiles, kza, n, lam, lam ^ 1, n - n0,
-                                        n - n0 + 1, rec_sample(ta, n), psum(n - n0 + 1),
+                                        rec_sample(ta, n),
                                         TileOrder{desc[0].ngroups, ctx->adjoint_tile_order},
                                         ctx->stream));
             lam ^= 1;
@@ -442,8 +453,12 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         std::vector<float> g(3 * N);
         HIP_TRY(hipMemcpy(g.data(), A.bufs[s]->lam + 8 * N, 3 * N * sizeof(float),
                           hipMemcpyDeviceToHost));
-        std::vector<float> gk(g.begin(), g.begin() + N), gbx(g.begin() + N, g.begin() + 2 * N),
-            gbz(g.begin() + 2 * N, g.end());
+        std::vector<float> gk(N), gbx(N), gbz(N);
+        for (size_t i = 0; i < N; i++) {
+            gk[i] = g[3 * i];
+            gbx[i] = g[3 * i + 1];
+            gbz[i] = g[3 * i + 2];
+        }
         chain_to_model(ctx, h, gk, gbx, gbz, g_vp, g_rho);
         if (want_seis && plans[s].ng)
             HIP_TRY(hipMemcpy(seis_out[s], plans[s].seis_dev,
@@ -529,14 +544,6 @@ extern "C" int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshot
         }
     }
     std::vector<double> gv((size_t)ctx->grid.nx * ctx->grid.nz, 0.0), gr(gv.size(), 0.0);
-    // The adjoint reads single fields of the forward states: the
-    // recomputation keeps one array per field (the forward sweep switches to
-    // the cell-interleaved layout for itself).
-    struct LayoutScope {
-        sam2d_ctx *c;
-        explicit LayoutScope(sam2d_ctx *x) : c(x) { c->state_aos = false; }
-        ~LayoutScope() { c->state_aos = true; }
-    } layout(ctx);
     for (int s0 = 0; s0 < nshots; s0 += batch) {
         const int s1 = std::min(nshots, s0 + batch);
         std::vector<ShotPlan> sub(plans.begin() + s0, plans.begin() + s1);

@@ -104,29 +104,64 @@ hipError_t launch_band(const int *bmap, const int *rptr, const int *rid, const f
     return hipGetLastError();
 }
 
+// Coefficient cells of the adjoint: KX, BX, BZ (KZ aliases KX) or KX, BX,
+// BZ, KZ, one dwordx3 or dwordx4 load per lane and column.
+__global__ void k_pack_coef(const float *__restrict__ kx, const float *__restrict__ kz,
+                            const float *__restrict__ bx, const float *__restrict__ bz, size_t n,
+                            float *__restrict__ out)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const int w = kz ? 4 : 3;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float *o = out + w * i;
+        o[0] = kx[i];
+        o[1] = bx[i];
+        o[2] = bz[i];
+        if (kz) o[3] = kz[i];
+    }
+}
+
+hipError_t launch_pack_coef(const float *kx, const float *kz, const float *bx, const float *bz,
+                            size_t n, float *out, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_pack_coef, dim3((unsigned)blocks), dim3(256), 0, stream, kx, kz, bx, bz,
+                       n, out);
+    return hipGetLastError();
+}
+
 // The adjoint state is cell-interleaved (L_px, L_pz, L_vx, L_vz of cell i at
 // 4i .. 4i+3): a column's state arrives as one 16-byte load per lane at
 // column c+1 (kept whole in the prefetch slot) and leaves as one 16-byte
-// store at the phase-B column, L_px and L_pz kept from phase A.
+// store at the phase-B column, L_px and L_pz kept from phase A.  The forward
+// triplet of step n (p[n], vx[n-1], vz[n-1]) and the coefficient cell (KX,
+// BX, BZ[, KZ]) arrive at column c+1 as well, their column-c parts waiting a
+// column in windows; the accumulator cell (dJ/dkappa, dJ/dbu2, dJ/dbu1)
+// arrives at column c and leaves with the adjoint state at the phase-B
+// column.
+template <bool KZA>
+using CoefCell = typename std::conditional<KZA, ftri, fquad>::type;
+
+template <bool KZA>
 struct AdjCol {
     fquad lam;                  // column c+1: adjoint state
-    float bx_n, fvx_n;          //             BX, fwd vx[n-1]
-    float bz;                   // column c:   BZ
-    float fvz, fpx, fpz;        //             fwd vz[n-1], px[n], pz[n]
-    float kx, kz, gk;           //             KX, KZ (unused when aliased), dJ/dkappa
-    float res;                  //             residual of this step (band rows)
-    float gbx_j, gbz_j;         // column j:   dJ/dbu2, dJ/dbu1
+    ftri fw;                    //             p[n], vx[n-1], vz[n-1]
+    CoefCell<KZA> cf;           //             KX, BX, BZ[, KZ]
+    ftri acc;                   // column c:   dJ/dkappa, dJ/dbu2, dJ/dbu1
+    float res;                  //             residual of this step
 };
 
 template <int WAVES, int PF, bool KZA>
 __global__ __launch_bounds__(WAVES * 64)
 void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
-               int ntiles, int step, int lam_in, int lam_out, int fwd_prev, int fwd_cur,
-               int rec_k, int psum, int ngroups, int xorder)
+               int ntiles, int step, int lam_in, int lam_out, int fslot, int rec_k, int ngroups,
+               int xorder)
 {
     static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
     using gcchar = const SAM2D_GLOBAL char;
     using gchar = SAM2D_GLOBAL char;
+    constexpr unsigned CW = KZA ? 3u : 4u;            // floats per coefficient cell
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
@@ -152,33 +187,17 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         const unsigned rc4 = rc * 4u;
         const size_t N = d.ncell;
 
-        // Adjoint state, forward ring slots, coefficients and accumulators
-        // never overlap.
+        // Adjoint state, forward ring, coefficients and accumulators never
+        // overlap.  The accumulators are read at a cell and written back there
+        // by the same lane only.
         gcchar *__restrict__ Lin = (gcchar *)(d.lam + 4 * (size_t)lam_in * N);
         gchar *__restrict__ Lout = (gchar *)(d.lam + 4 * (size_t)lam_out * N);
-        // psum & 1: slot fwd_cur holds p in its px field and no pz; the pz
-        // loads then read the px field again (a cache hit) and are not used.
-        const bool ps_cur = psum & 1;
-        gcchar *__restrict__ Fpx = (gcchar *)(d.fwd + (4 * (size_t)fwd_cur + 0) * N);
-        gcchar *__restrict__ Fpz = (gcchar *)(d.fwd + (4 * (size_t)fwd_cur + (ps_cur ? 0 : 1)) * N);
-        gcchar *__restrict__ Fvx = (gcchar *)(d.fwd + (4 * (size_t)fwd_prev + 2) * N);
-        gcchar *__restrict__ Fvz = (gcchar *)(d.fwd + (4 * (size_t)fwd_prev + 3) * N);
-        gcchar *__restrict__ KX = (gcchar *)d.KX;
-        gcchar *__restrict__ KZ = (gcchar *)d.KZ;
-        gcchar *__restrict__ BX = (gcchar *)d.BX;
-        gcchar *__restrict__ BZ = (gcchar *)d.BZ;
-        // The accumulators are read at a column and written back at the same
-        // cell by the same lane only.
-        gchar *__restrict__ GK = (gchar *)d.gk;
-        gchar *__restrict__ GBX = (gchar *)d.gbx;
-        gchar *__restrict__ GBZ = (gchar *)d.gbz;
+        gcchar *__restrict__ FW = (gcchar *)(d.ring + 3 * (size_t)fslot * N);
+        gcchar *__restrict__ CF = (gcchar *)d.coef;
+        gchar *__restrict__ ACC = (gchar *)d.acc;
         const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);
         const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
 
-        auto ld = [](gcchar *__restrict__ base, unsigned off) -> float {
-            return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
-        };
-        // Adjoint state: byte offset 4x that of one field.
         auto ld4 = [](gcchar *__restrict__ base, unsigned off) -> fquad {
             return *(const SAM2D_GLOBAL fquad *__restrict__)(base + off);
         };
@@ -187,12 +206,12 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             v.x = a; v.y = b; v.z = c; v.w = e;
             *(SAM2D_GLOBAL fquad *__restrict__)(base + off) = v;
         };
-        auto ldw = [](gchar *__restrict__ base, unsigned off) -> float {
-            return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
+        auto ldcf = [&](unsigned off) -> CoefCell<KZA> {
+            if constexpr (KZA) return ld3(CF, 3 * off);
+            else return ld4(CF, 4 * off);
         };
-        auto stf = [](gchar *__restrict__ base, unsigned off, float val) {
-            *(SAM2D_GLOBAL float *__restrict__)(base + off) = val;
-        };
+        // Byte offset of this lane's row in column col of a one-float field
+        // (clamped to the grid); a cell of w floats is at w times it.
         auto coff = [&](int col) -> unsigned {
             return (unsigned)min(max(col, 0), nnx - 1) * (unsigned)nnz * 4u + rc4;
         };
@@ -221,16 +240,13 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             tile_band ? gp(d.rband) + ((ptrdiff_t)rec_k * d.band_h - d.rrow_lo) * (ptrdiff_t)nnx
                       : nullptr;
 
-        auto load_col = [&](int c) -> AdjCol {
-            AdjCol o;
-            const unsigned o1 = coff(c + 1), o0 = coff(c), oj = coff(c - 2);
-            o.lam = ld4(Lin, 4 * o1); o.bx_n = ld(BX, o1); o.fvx_n = ld(Fvx, o1);
-            o.bz = ld(BZ, o0);
-            o.fvz = ld(Fvz, o0); o.fpx = ld(Fpx, o0); o.fpz = ld(Fpz, o0);
-            o.kx = ld(KX, o0);
-            o.kz = KZA ? 0.0f : ld(KZ, o0);
-            o.gk = ldw(GK, o0);
-            o.gbx_j = ldw(GBX, oj); o.gbz_j = ldw(GBZ, oj);
+        auto load_col = [&](int c) -> AdjCol<KZA> {
+            AdjCol<KZA> o;
+            const unsigned o1 = coff(c + 1), o0 = coff(c);
+            o.lam = ld4(Lin, 4 * o1);
+            o.fw = ld3(FW, 3 * o1);
+            o.cf = ldcf(o1);
+            o.acc = ld3(ACC, 3 * o0);
             o.res = -0.0f;
             if (tile_band) {
                 if (band_lane) o.res = rb[(size_t)r * nnx + clampc(c)];
@@ -248,13 +264,17 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         };
 
         // vx-style windows: [k & 3] = column c-2+k (LVX, LPX, LPZ, LVZ: the
-        // input adjoint state).  p-style: [k & 3] = column c-3+k (OPX, OPZ:
-        // the new L_px, L_pz, stored with phase B).
-        float LVX[4], LPX[4], LPZ[4], LVZ[4], FVX[4], AXP[4], BXP[4];
+        // input adjoint state; FVX, FVZ, FP: the forward triplet; KXn, KZn,
+        // BZn: coefficients; GBX, GBZ: accumulators as loaded; GK: dJ/dkappa
+        // after phase A).  p-style: [k & 3] = column c-3+k (OPX, OPZ: the new
+        // L_px, L_pz, stored with phase B).
+        float LVX[4], LPX[4], LPZ[4], LVZ[4], FVX[4], FVZ[4], FP[4], AXP[4], BXP[4];
+        float KXn[4], KZn[4], BZn[4], GBX[4], GBZ[4], GK[4];
         float AM[4], BM[4], AMZ[4], BMZ[4], PN[4], OPX[4], OPZ[4];
         #pragma unroll
         for (int k = 0; k < 4; k++) {
-            LVX[k] = LPX[k] = LPZ[k] = LVZ[k] = FVX[k] = AXP[k] = BXP[k] = 0.0f;
+            LVX[k] = LPX[k] = LPZ[k] = LVZ[k] = FVX[k] = FVZ[k] = FP[k] = AXP[k] = BXP[k] = 0.0f;
+            KXn[k] = KZn[k] = BZn[k] = GBX[k] = GBZ[k] = GK[k] = 0.0f;
             AM[k] = BM[k] = AMZ[k] = BMZ[k] = PN[k] = OPX[k] = OPZ[k] = 0.0f;
         }
         #pragma unroll
@@ -263,11 +283,15 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             const unsigned o = coff(col);
             const fquad lv = ld4(Lin, 4 * o);
             LPX[k] = lv.x; LPZ[k] = lv.y; LVX[k] = lv.z; LVZ[k] = lv.w;
-            FVX[k] = ld(Fvx, o);
-            gx_terms(col, LVX[k], ld(BX, o), AXP[k], BXP[k]);
+            const ftri fw = ld3(FW, 3 * o);
+            FP[k] = fw.x; FVX[k] = fw.y; FVZ[k] = fw.z;
+            const CoefCell<KZA> cf = ldcf(o);
+            KXn[k] = cf.x; BZn[k] = cf.z;
+            if constexpr (!KZA) KZn[k] = cf.w;
+            gx_terms(col, LVX[k], cf.y, AXP[k], BXP[k]);
         }
 
-        AdjCol q[PF];
+        AdjCol<KZA> q[PF];
         #pragma unroll
         for (int u = 0; u < PF; u++) q[u] = load_col(xa - 1 + u);
 
@@ -276,7 +300,7 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             constexpr int u = decltype(u_tag)::value;
             constexpr int i0 = u & 3, i1 = (u + 1) & 3, i2 = (u + 2) & 3, i3 = (u + 3) & 3;
             const int c = c0 + u;
-            AdjCol &cur = q[u % PF];
+            AdjCol<KZA> &cur = q[u % PF];
 
             // ---- phase A at column c
             LPX[i3] = keep(cur.lam.x);
@@ -284,16 +308,24 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             LVZ[i3] = keep(cur.lam.w);
             const float lvx_n = keep(cur.lam.z);
             LVX[i3] = lvx_n;
-            FVX[i3] = keep(cur.fvx_n);
+            FP[i3] = keep(cur.fw.x);
+            FVX[i3] = keep(cur.fw.y);
+            FVZ[i3] = keep(cur.fw.z);
+            const float bx_n = keep(cur.cf.y);
+            KXn[i3] = keep(cur.cf.x);
+            BZn[i3] = keep(cur.cf.z);
+            if constexpr (!KZA) KZn[i3] = keep(cur.cf.w);
+            GBX[i2] = keep(cur.acc.y);
+            GBZ[i2] = keep(cur.acc.z);
             if (FAST) {
-                const float g = cur.bx_n * lvx_n;
+                const float g = bx_n * lvx_n;
                 AXP[i3] = SAM2D_A0 * g;
                 BXP[i3] = SAM2D_A1 * g;
             } else {
-                gx_terms(c + 1, lvx_n, cur.bx_n, AXP[i3], BXP[i3]);
+                gx_terms(c + 1, lvx_n, bx_n, AXP[i3], BXP[i3]);
             }
             const float ax = AXP[i1] - AXP[i2] + (BXP[i0] - BXP[i3]);
-            const float gz = (cur.bz * sbz) * LVZ[i2];
+            const float gz = (BZn[i2] * sbz) * LVZ[i2];
             const float az = zp0 * gz, bzv = zp1 * gz;
             const float bz_u1 = lane_up1(bzv);
             const float azz = (lane_up1(az) - az) + (lane_up1(bz_u1) - lane_dn1(bzv));
@@ -304,7 +336,7 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             if (!FAST) dminus_coef(c, nnx, xm0, xm1);
             // dJ/dkappa from the p-update needs Dx- vx[n-1] and Dz- vz[n-1];
             // the lane shifts run with every lane active.
-            const float f0 = cur.fvz;
+            const float f0 = FVZ[i2];
             const float f_u1 = lane_up1(f0), f_u2 = lane_up1(f_u1), f_d1 = lane_dn1(f0);
             float skx = 1.0f, dax = 1.0f;
             if (!FAST) {
@@ -312,24 +344,24 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
                 dax = prx[cc];
                 skx = prx[nnx + cc];
             }
-            if (rown && c >= xa && c < xb) {
+            {
                 const float dvx = fmaf(xm0, FVX[i2] - FVX[i1], xm1 * (FVX[i3] - FVX[i0]));
                 const float dvz = fmaf(zm0, f0 - f_u1, zm1 * (f_d1 - f_u2));
-                float g = cur.gk + lpx1 * dvx * (dtdx * skx) + lpz1 * dvz * (dtdz * skz);
+                float g = cur.acc.x + lpx1 * dvx * (dtdx * skx) + lpz1 * dvz * (dtdz * skz);
                 if (src_lane && c == src_x) g += lpx1 * srcg;
-                const unsigned id = (unsigned)c * (unsigned)nnz * 4u + rc4;
-                stf(GK, id, g);
+                GK[i2] = g;
             }
             OPX[i3] = dax * lpx1;
             OPZ[i3] = daz * lpz1;
-            const float kz = KZA ? cur.kx : cur.kz;
-            const float hx = (cur.kx * skx) * lpx1;
+            const float kx = KXn[i2];
+            const float kz = KZA ? kx : KZn[i2];
+            const float hx = (kx * skx) * lpx1;
             const float hz = (kz * skz) * lpz1;
             AM[i3] = xm0 * hx;
             BM[i3] = xm1 * hx;
             AMZ[i3] = zm0 * hz;
             BMZ[i3] = zm1 * hz;
-            PN[i3] = ps_cur ? cur.fpx : cur.fpx + cur.fpz;
+            PN[i3] = FP[i2];
 
             // ---- phase B at column j = c - 2
             const int j = c - 2;
@@ -353,8 +385,8 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
                 const unsigned id = (unsigned)j * (unsigned)nnz * 4u + rc4;
                 const float lvz_j = LVZ[i0];
                 st4(Lout, 4 * id, OPX[i1], OPZ[i1], fmaf(dbx, lvx_j, tx), fmaf(dbz, lvz_j, tz));
-                stf(GBX, id, cur.gbx_j + lvx_j * dpx * (dtdx * sbx));
-                stf(GBZ, id, cur.gbz_j + lvz_j * dpz * (dtdz * sbz));
+                st3(ACC, 3 * id, GK[i0], GBX[i0] + lvx_j * dpx * (dtdx * sbx),
+                    GBZ[i0] + lvz_j * dpz * (dtdz * sbz));
             }
             cur = load_col(min(c + PF, c_last));   // past c_last: a cache hit
         };
@@ -398,25 +430,25 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
 // with the same arithmetic per cell as k_adjoint, so both give the same bits.
 // The x-transposes are formed at use from windows of the per-column factors
 // (G = BXe L_vx, H = KXe L_px', HZ = KZe L_pz') and per-index coefficients.
-// Forward states: vx, vz after step n-2 in ring slot fa, after step n-1 in fb;
-// px, pz after step n-1 in fb, after step n in fc.
+// Forward triplets: step n's (p[n], vx[n-1], vz[n-1]) in ring slot fslot,
+// loaded at column c+1; step n-1's (p[n-1], vx[n-2], vz[n-2]) in fslot - 1,
+// loaded at column c-2.
+template <bool KZA>
 struct AdjCol2 {
     fquad lam;                  // column c+1: adjoint state (cell-interleaved)
-    float bx_n, fvx_n;          //             BX, fwd vx[n-1]
-    float bz;                   // column c:   BZ
-    float fvz, fpx, fpz;        //             fwd vz[n-1], px[n], pz[n]
-    float kx, kz, gk, res1;     //             KX, KZ, dJ/dkappa, residual of step n
-    float gbx_j, gbz_j;         // column c-2: dJ/dbu2, dJ/dbu1
-    float fvx2;                 //             fwd vx[n-2]
-    float fvz2, fpx2, fpz2;     // column c-3: fwd vz[n-2], px[n-1], pz[n-1]
-    float res2;                 //             residual of step n-1
+    ftri fw1;                   //             forward triplet of step n
+    CoefCell<KZA> cf;           //             KX, BX, BZ[, KZ]
+    ftri acc;                   // column c:   dJ/dkappa, dJ/dbu2, dJ/dbu1
+    float res1;                 //             residual of step n
+    ftri fw2;                   // column c-2: forward triplet of step n-1
+    float res2;                 // column c-3: residual of step n-1
 };
 
 template <int WAVES, int PF, bool KZA>
 __global__ __launch_bounds__(WAVES * 64)
 void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
-                int ntiles, int step, int lam_in, int lam_out, int fa, int fb, int fc,
-                int rec1, int rec2, int psum, int ngroups, int xorder)
+                int ntiles, int step, int lam_in, int lam_out, int fslot, int rec1, int rec2,
+                int ngroups, int xorder)
 {
     static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
     using gcchar = const SAM2D_GLOBAL char;
@@ -448,30 +480,13 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
 
         gcchar *__restrict__ Lin = (gcchar *)(d.lam + 4 * (size_t)lam_in * N);
         gchar *__restrict__ Lout = (gchar *)(d.lam + 4 * (size_t)lam_out * N);
-        // psum bit 0 (slot fc) and bit 1 (slot fb): the slot holds p in its px
-        // field and no pz; the pz loads then read the px field again.
-        const bool ps_c = psum & 1, ps_b = (psum >> 1) & 1;
-        gcchar *__restrict__ Fpx1 = (gcchar *)(d.fwd + (4 * (size_t)fc + 0) * N);
-        gcchar *__restrict__ Fpz1 = (gcchar *)(d.fwd + (4 * (size_t)fc + (ps_c ? 0 : 1)) * N);
-        gcchar *__restrict__ Fvx1 = (gcchar *)(d.fwd + (4 * (size_t)fb + 2) * N);
-        gcchar *__restrict__ Fvz1 = (gcchar *)(d.fwd + (4 * (size_t)fb + 3) * N);
-        gcchar *__restrict__ Fpx2 = (gcchar *)(d.fwd + (4 * (size_t)fb + 0) * N);
-        gcchar *__restrict__ Fpz2 = (gcchar *)(d.fwd + (4 * (size_t)fb + (ps_b ? 0 : 1)) * N);
-        gcchar *__restrict__ Fvx2 = (gcchar *)(d.fwd + (4 * (size_t)fa + 2) * N);
-        gcchar *__restrict__ Fvz2 = (gcchar *)(d.fwd + (4 * (size_t)fa + 3) * N);
-        gcchar *__restrict__ KX = (gcchar *)d.KX;
-        gcchar *__restrict__ KZ = (gcchar *)d.KZ;
-        gcchar *__restrict__ BX = (gcchar *)d.BX;
-        gcchar *__restrict__ BZ = (gcchar *)d.BZ;
-        gchar *__restrict__ GK = (gchar *)d.gk;
-        gchar *__restrict__ GBX = (gchar *)d.gbx;
-        gchar *__restrict__ GBZ = (gchar *)d.gbz;
+        gcchar *__restrict__ FW1 = (gcchar *)(d.ring + 3 * (size_t)fslot * N);
+        gcchar *__restrict__ FW2 = (gcchar *)(d.ring + 3 * (size_t)(fslot - 1) * N);
+        gcchar *__restrict__ CF = (gcchar *)d.coef;
+        gchar *__restrict__ ACC = (gchar *)d.acc;
         const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);
         const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
 
-        auto ld = [](gcchar *__restrict__ base, unsigned off) -> float {
-            return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
-        };
         // Adjoint state: byte offset 4x that of one field.
         auto ld4 = [](gcchar *__restrict__ base, unsigned off) -> fquad {
             return *(const SAM2D_GLOBAL fquad *__restrict__)(base + off);
@@ -481,11 +496,9 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
             v.x = a; v.y = b; v.z = c; v.w = e;
             *(SAM2D_GLOBAL fquad *__restrict__)(base + off) = v;
         };
-        auto ldw = [](gchar *__restrict__ base, unsigned off) -> float {
-            return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
-        };
-        auto stf = [](gchar *__restrict__ base, unsigned off, float val) {
-            *(SAM2D_GLOBAL float *__restrict__)(base + off) = val;
+        auto ldcf = [&](unsigned off) -> CoefCell<KZA> {
+            if constexpr (KZA) return ld3(CF, 3 * off);
+            else return ld4(CF, 4 * off);
         };
         auto coff = [&](int col) -> unsigned {
             return (unsigned)min(max(col, 0), nnx - 1) * (unsigned)nnz * 4u + rc4;
@@ -523,18 +536,14 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
             band2 ? gp(d.rband) + ((ptrdiff_t)rec2 * d.band_h - d.rrow_lo) * (ptrdiff_t)nnx
                   : nullptr;
 
-        auto load_col = [&](int c) -> AdjCol2 {
-            AdjCol2 o;
-            const unsigned o1 = coff(c + 1), o0 = coff(c), oj = coff(c - 2), o3 = coff(c - 3);
-            o.lam = ld4(Lin, 4 * o1); o.bx_n = ld(BX, o1); o.fvx_n = ld(Fvx1, o1);
-            o.bz = ld(BZ, o0);
-            o.fvz = ld(Fvz1, o0); o.fpx = ld(Fpx1, o0); o.fpz = ld(Fpz1, o0);
-            o.kx = ld(KX, o0);
-            o.kz = KZA ? 0.0f : ld(KZ, o0);
-            o.gk = ldw(GK, o0);
-            o.gbx_j = ldw(GBX, oj); o.gbz_j = ldw(GBZ, oj);
-            o.fvx2 = ld(Fvx2, oj);
-            o.fvz2 = ld(Fvz2, o3); o.fpx2 = ld(Fpx2, o3); o.fpz2 = ld(Fpz2, o3);
+        auto load_col = [&](int c) -> AdjCol2<KZA> {
+            AdjCol2<KZA> o;
+            const unsigned o1 = coff(c + 1), o0 = coff(c), oj = coff(c - 2);
+            o.lam = ld4(Lin, 4 * o1);
+            o.fw1 = ld3(FW1, 3 * o1);
+            o.cf = ldcf(o1);
+            o.acc = ld3(ACC, 3 * o0);
+            o.fw2 = ld3(FW2, 3 * oj);
             o.res1 = -0.0f;
             o.res2 = -0.0f;
             if (band1) {
@@ -548,22 +557,28 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
 
         // Windows, [k & 3] relative to the unrolled base (columns held):
         //   G1 LVX0 FVX1 BXW                      c-2 .. c+1
+        //   FP1n FVZ1n KXn KZn BZn (as loaded)    c-2 .. c+1
         //   LVZ0 BZW KXW KZW LPX1 LPZ1 GKW H1 HZ1 PN1   c-3 .. c
+        //   GBXn GBZn (accumulators as loaded)    c-3 .. c
         //   G2 LVX1 LVZ1 GBXW GBZW FVX2           c-5 .. c-2
-        //   H2 HZ2 PN2                            c-6 .. c-3
+        //   FP2n FVZ2n (as loaded)                c-5 .. c-2
+        //   H2 HZ2 PN2 GK2                        c-6 .. c-3
         //   LPXn LPZn LVZn (input L_px, L_pz, L_vz)  c-2 .. c+1
         //   OPA OPZA (new L_px, L_pz of step n-1)   c-6 .. c-3, stored with B2
         float G1[4], LVX0[4], FVX1[4], BXW[4], LPXn[4], LPZn[4], LVZn[4];
+        float FP1n[4], FVZ1n[4], KXn[4], KZn[4], BZn[4], GBXn[4], GBZn[4], FP2n[4], FVZ2n[4];
         float LVZ0[4], BZW[4], KXW[4], KZW[4], LPX1[4], LPZ1[4], GKW[4], H1[4], HZ1[4], PN1[4];
         float G2[4], LVX1[4], LVZ1[4], GBXW[4], GBZW[4], FVX2[4];
-        float H2[4], HZ2[4], PN2[4], OPA[4], OPZA[4];
+        float H2[4], HZ2[4], PN2[4], GK2[4], OPA[4], OPZA[4];
         #pragma unroll
         for (int k = 0; k < 4; k++) {
             G1[k] = LVX0[k] = FVX1[k] = BXW[k] = LPXn[k] = LPZn[k] = LVZn[k] = 0.0f;
+            FP1n[k] = FVZ1n[k] = KXn[k] = KZn[k] = BZn[k] = GBXn[k] = GBZn[k] = 0.0f;
+            FP2n[k] = FVZ2n[k] = 0.0f;
             LVZ0[k] = BZW[k] = KXW[k] = KZW[k] = LPX1[k] = LPZ1[k] = GKW[k] = 0.0f;
             H1[k] = HZ1[k] = PN1[k] = 0.0f;
             G2[k] = LVX1[k] = LVZ1[k] = GBXW[k] = GBZW[k] = FVX2[k] = 0.0f;
-            H2[k] = HZ2[k] = PN2[k] = OPA[k] = OPZA[k] = 0.0f;
+            H2[k] = HZ2[k] = PN2[k] = GK2[k] = OPA[k] = OPZA[k] = 0.0f;
         }
         const int c_first = xa - 4;
         #pragma unroll
@@ -572,12 +587,15 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
             const unsigned o = coff(col);
             const fquad lv = ld4(Lin, 4 * o);
             LPXn[k] = lv.x; LPZn[k] = lv.y; LVX0[k] = lv.z; LVZn[k] = lv.w;
-            FVX1[k] = ld(Fvx1, o);
-            BXW[k] = ld(BX, o);
+            const ftri fw = ld3(FW1, 3 * o);
+            FP1n[k] = fw.x; FVX1[k] = fw.y; FVZ1n[k] = fw.z;
+            const CoefCell<KZA> cf = ldcf(o);
+            KXn[k] = cf.x; BXW[k] = cf.y; BZn[k] = cf.z;
+            if constexpr (!KZA) KZn[k] = cf.w;
             G1[k] = (BXW[k] * prx[3 * nnx + clampc(col)]) * LVX0[k];
         }
 
-        AdjCol2 q[PF];
+        AdjCol2<KZA> q[PF];
         #pragma unroll
         for (int u = 0; u < PF; u++) q[u] = load_col(c_first + u);
 
@@ -586,7 +604,7 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
             constexpr int u = decltype(u_tag)::value;
             constexpr int i0 = u & 3, i1 = (u + 1) & 3, i2 = (u + 2) & 3, i3 = (u + 3) & 3;
             const int c = c0 + u;
-            AdjCol2 &cur = q[u % PF];
+            AdjCol2<KZA> &cur = q[u % PF];
 
             // ---- A1: step n, phase A at column c
             {
@@ -595,9 +613,16 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                 LVZn[i3] = keep(cur.lam.w);
                 const float lvx_n = keep(cur.lam.z);
                 LVX0[i3] = lvx_n;
-                FVX1[i3] = keep(cur.fvx_n);
-                const float bxn = keep(cur.bx_n);
+                FP1n[i3] = keep(cur.fw1.x);
+                FVX1[i3] = keep(cur.fw1.y);
+                FVZ1n[i3] = keep(cur.fw1.z);
+                const float bxn = keep(cur.cf.y);
                 BXW[i3] = bxn;
+                KXn[i3] = keep(cur.cf.x);
+                BZn[i3] = keep(cur.cf.z);
+                if constexpr (!KZA) KZn[i3] = keep(cur.cf.w);
+                GBXn[i3] = keep(cur.acc.y);
+                GBZn[i3] = keep(cur.acc.z);
                 G1[i3] = FAST ? bxn * lvx_n : (bxn * prx[3 * nnx + clampc(c + 1)]) * lvx_n;
                 float ax;
                 if (FAST) {
@@ -609,7 +634,7 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                 }
                 const float lvz = LVZn[i2];
                 LVZ0[i3] = lvz;
-                const float bz = keep(cur.bz);
+                const float bz = BZn[i2];
                 BZW[i3] = bz;
                 const float gz = (bz * sbz) * lvz;
                 const float az = zp0 * gz, bzv = zp1 * gz;
@@ -625,22 +650,22 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                     dax = prx[cc];
                     skx = prx[nnx + cc];
                 }
-                const float f0 = cur.fvz;
+                const float f0 = FVZ1n[i2];
                 const float f_u1 = lane_up1(f0), f_u2 = lane_up1(f_u1), f_d1 = lane_dn1(f0);
                 const float dvx = fmaf(xm0, FVX1[i2] - FVX1[i1], xm1 * (FVX1[i3] - FVX1[i0]));
                 const float dvz = fmaf(zm0, f0 - f_u1, zm1 * (f_d1 - f_u2));
-                float g = cur.gk + lpx1 * dvx * (dtdx * skx) + lpz1 * dvz * (dtdz * skz);
+                float g = cur.acc.x + lpx1 * dvx * (dtdx * skx) + lpz1 * dvz * (dtdz * skz);
                 if (src_lane && c == src_x1) g += lpx1 * srcg1;
                 GKW[i3] = g;
                 LPX1[i3] = dax * lpx1;
                 LPZ1[i3] = daz * lpz1;
-                const float kx = keep(cur.kx);
-                const float kz = KZA ? kx : keep(cur.kz);
+                const float kx = KXn[i2];
+                const float kz = KZA ? kx : KZn[i2];
                 KXW[i3] = kx;
                 if (!KZA) KZW[i3] = kz;
                 H1[i3] = (kx * skx) * lpx1;
                 HZ1[i3] = (kz * skz) * lpz1;
-                PN1[i3] = ps_c ? cur.fpx : cur.fpx + cur.fpz;
+                PN1[i3] = FP1n[i2];
             }
 
             // ---- B1: step n, phase B at column j1 = c-2
@@ -672,10 +697,12 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                 const float ovx = fmaf(dbx, lvx_j, tx);
                 LVX1[i3] = ovx;
                 LVZ1[i3] = fmaf(dbz, lvz_j, tz);
-                GBXW[i3] = keep(cur.gbx_j) + lvx_j * dpx * (dtdx * sbx);
-                GBZW[i3] = keep(cur.gbz_j) + lvz_j * dpz * (dtdz * sbz);
+                GBXW[i3] = GBXn[i1] + lvx_j * dpx * (dtdx * sbx);
+                GBZW[i3] = GBZn[i1] + lvz_j * dpz * (dtdz * sbz);
                 G2[i3] = (BXW[i0] * sbx) * ovx;
-                FVX2[i3] = keep(cur.fvx2);
+                FP2n[i3] = keep(cur.fw2.x);
+                FVX2[i3] = keep(cur.fw2.y);
+                FVZ2n[i3] = keep(cur.fw2.z);
             }
 
             // ---- A2: step n-1, phase A at column c2 = c-3 (results stored)
@@ -703,22 +730,21 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                     dax = prx[cc];
                     skx = prx[nnx + cc];
                 }
-                const float f0 = cur.fvz2;
+                const float f0 = FVZ2n[i2];
                 const float f_u1 = lane_up1(f0), f_u2 = lane_up1(f_u1), f_d1 = lane_dn1(f0);
-                if (rown && c2 >= xa && c2 < xb) {
+                {
                     const float dvx = fmaf(xm0, FVX2[i2] - FVX2[i1], xm1 * (FVX2[i3] - FVX2[i0]));
                     const float dvz = fmaf(zm0, f0 - f_u1, zm1 * (f_d1 - f_u2));
                     float g = GKW[i0] + lpx2 * dvx * (dtdx * skx) + lpz2 * dvz * (dtdz * skz);
                     if (src_lane && c2 == src_x2) g += lpx2 * srcg2;
-                    const unsigned id = (unsigned)c2 * (unsigned)nnz * 4u + rc4;
-                    stf(GK, id, g);
+                    GK2[i3] = g;                  // stored with phase B2
                 }
                 OPA[i3] = dax * lpx2;
                 OPZA[i3] = daz * lpz2;
                 const float kz = KZA ? KXW[i0] : KZW[i0];
                 H2[i3] = (KXW[i0] * skx) * lpx2;
                 HZ2[i3] = (kz * skz) * lpz2;
-                PN2[i3] = ps_b ? cur.fpx2 : cur.fpx2 + cur.fpz2;
+                PN2[i3] = FP2n[i2];
             }
 
             // ---- B2: step n-1, phase B at column j2 = c-5 (results stored)
@@ -750,8 +776,8 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                     const float dpz = fmaf(zp0, q_d1 - pj, zp1 * (q_d2 - q_u1));
                     const unsigned id = (unsigned)j2 * (unsigned)nnz * 4u + rc4;
                     st4(Lout, 4 * id, OPA[i1], OPZA[i1], fmaf(dbx, lvx_j, tx), fmaf(dbz, lvz_j, tz));
-                    stf(GBX, id, GBXW[i0] + lvx_j * dpx * (dtdx * sbx));
-                    stf(GBZ, id, GBZW[i0] + lvz_j * dpz * (dtdz * sbz));
+                    st3(ACC, 3 * id, GK2[i1], GBXW[i0] + lvx_j * dpx * (dtdx * sbx),
+                        GBZW[i0] + lvz_j * dpz * (dtdz * sbz));
                 }
             }
             cur = load_col(min(c + PF, c_last));   // past c_last: a cache hit
@@ -785,83 +811,65 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
     }
 }
 
+struct AdjArgs {
+    const AdjDev *d;
+    int ns, vt, per, nt, step, li, lo, fslot, r1, r2;
+    TileOrder o;
+};
+
 template <int W, int PF, bool KZA>
-static void launch_adj_t(dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt, int per, int nt,
-                         int step, int li, int lo, int fp, int fc, int rk, int ps, TileOrder o)
+static void launch_adj_t(dim3 g, hipStream_t s, const AdjArgs &a, bool two)
 {
-    hipLaunchKernelGGL((k_adjoint<W, PF, KZA>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step, li,
-                       lo, fp, fc, rk, ps, o.ngroups, o.xorder);
+    if (two)
+        hipLaunchKernelGGL((k_adjoint2<W, PF, KZA>), g, dim3(64 * W), 0, s, a.d, a.ns, a.vt, a.per,
+                           a.nt, a.step, a.li, a.lo, a.fslot, a.r1, a.r2, a.o.ngroups, a.o.xorder);
+    else
+        hipLaunchKernelGGL((k_adjoint<W, PF, KZA>), g, dim3(64 * W), 0, s, a.d, a.ns, a.vt, a.per,
+                           a.nt, a.step, a.li, a.lo, a.fslot, a.r1, a.o.ngroups, a.o.xorder);
 }
 
 template <int W>
-static void launch_adj_w(int pf, bool kza, dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt,
-                         int per, int nt, int step, int li, int lo, int fp, int fc, int rk,
-                         int ps, TileOrder o)
+static void launch_adj_w(int pf, bool kza, dim3 g, hipStream_t s, const AdjArgs &a, bool two)
 {
-    if (pf == 2 && kza) launch_adj_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps, o);
-    else if (pf == 2) launch_adj_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps, o);
-    else if (kza) launch_adj_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps, o);
-    else launch_adj_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fp, fc, rk, ps, o);
+    if (pf == 2 && kza) launch_adj_t<W, 2, true>(g, s, a, two);
+    else if (pf == 2) launch_adj_t<W, 2, false>(g, s, a, two);
+    else if (kza) launch_adj_t<W, 4, true>(g, s, a, two);
+    else launch_adj_t<W, 4, false>(g, s, a, two);
+}
+
+static hipError_t launch_adj(int variant, bool kz_alias, bool two, hipStream_t stream, AdjArgs a)
+{
+    int waves, chunk, pf;
+    variant_shape_pf(variant, &waves, &chunk, &pf);
+    a.per = per_xcd_for(a.nt, a.o);
+    a.vt = a.per * 8;
+    const dim3 g(a.vt);
+    switch (waves) {
+    case 1: launch_adj_w<1>(pf, kz_alias, g, stream, a, two); break;
+    case 2: launch_adj_w<2>(pf, kz_alias, g, stream, a, two); break;
+    case 4: launch_adj_w<4>(pf, kz_alias, g, stream, a, two); break;
+    case 8: launch_adj_w<8>(pf, kz_alias, g, stream, a, two); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
-                               bool kz_alias, int step, int lam_in, int lam_out, int fwd_prev,
-                               int fwd_cur, int rec_k, int psum, TileOrder order,
-                               hipStream_t stream)
+                               bool kz_alias, int step, int lam_in, int lam_out, int fslot,
+                               int rec_k, TileOrder order, hipStream_t stream)
 {
-    int waves, chunk, pf;
-    variant_shape_pf(variant, &waves, &chunk, &pf);
-    const int per = per_xcd_for(ntiles, order);
-    const int vtotal = per * 8;
-    const dim3 g(vtotal);
-    switch (waves) {
-    case 1: launch_adj_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum, order); break;
-    case 2: launch_adj_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum, order); break;
-    case 4: launch_adj_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum, order); break;
-    case 8: launch_adj_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fwd_prev, fwd_cur, rec_k, psum, order); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-template <int W, int PF, bool KZA>
-static void launch_adj2_t(dim3 g, hipStream_t s, const AdjDev *d, int ns, int vt, int per,
-                          int nt, int step, int li, int lo, int fa, int fb, int fc, int r1, int r2,
-                          int ps, TileOrder o)
-{
-    hipLaunchKernelGGL((k_adjoint2<W, PF, KZA>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step,
-                       li, lo, fa, fb, fc, r1, r2, ps, o.ngroups, o.xorder);
-}
-
-template <int W>
-static void launch_adj2_w(int pf, bool kza, dim3 g, hipStream_t s, const AdjDev *d, int ns,
-                          int vt, int per, int nt, int step, int li, int lo, int fa, int fb, int fc,
-                          int r1, int r2, int ps, TileOrder o)
-{
-    if (pf == 2 && kza) launch_adj2_t<W, 2, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps, o);
-    else if (pf == 2) launch_adj2_t<W, 2, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps, o);
-    else if (kza) launch_adj2_t<W, 4, true>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps, o);
-    else launch_adj2_t<W, 4, false>(g, s, d, ns, vt, per, nt, step, li, lo, fa, fb, fc, r1, r2, ps, o);
+    return launch_adj(variant, kz_alias, false, stream,
+                      AdjArgs{d_adj, nshots, 0, 0, ntiles, step, lam_in, lam_out, fslot, rec_k, -1,
+                              order});
 }
 
 hipError_t launch_adjoint_step2(int variant, const AdjDev *d_adj, int nshots, int ntiles,
-                                bool kz_alias, int step, int lam_in, int lam_out, int fa, int fb,
-                                int fc, int rec1, int rec2, int psum, TileOrder order,
-                                hipStream_t stream)
+                                bool kz_alias, int step, int lam_in, int lam_out, int fslot,
+                                int rec1, int rec2, TileOrder order, hipStream_t stream)
 {
-    int waves, chunk, pf;
-    variant_shape_pf(variant, &waves, &chunk, &pf);
-    const int per = per_xcd_for(ntiles, order);
-    const int vtotal = per * 8;
-    const dim3 g(vtotal);
-    switch (waves) {
-    case 1: launch_adj2_w<1>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum, order); break;
-    case 2: launch_adj2_w<2>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum, order); break;
-    case 4: launch_adj2_w<4>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum, order); break;
-    case 8: launch_adj2_w<8>(pf, kz_alias, g, stream, d_adj, nshots, vtotal, per, ntiles, step, lam_in, lam_out, fa, fb, fc, rec1, rec2, psum, order); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return launch_adj(variant, kz_alias, true, stream,
+                      AdjArgs{d_adj, nshots, 0, 0, ntiles, step, lam_in, lam_out, fslot, rec1, rec2,
+                              order});
 }
 
 hipError_t launch_inject(const AdjDev *d_adj, int nshots, int max_cells, int lam, int rec_k,

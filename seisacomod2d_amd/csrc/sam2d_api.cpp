@@ -67,7 +67,7 @@ int upload(void *dst, const void *src, size_t bytes, hipStream_t s)
 DeviceDomain::~DeviceDomain()
 {
     dev_free(KX); if (KZ != KX) dev_free(KZ);
-    dev_free(BX); dev_free(BZ); dev_free(B); dev_free(prof_x); dev_free(prof_z);
+    dev_free(BX); dev_free(BZ); dev_free(B); dev_free(prof_x); dev_free(prof_z); dev_free(ADJ);
 }
 
 ShotBuffers::~ShotBuffers() { release(); }
@@ -264,7 +264,7 @@ void describe_shot(const ShotPlan &p, int variant, int chunk, int tile_begin, in
 // cell-interleaved state, 4 N per field.
 int check_multistep_size(const sam2d_ctx *ctx, int variant, size_t ncell)
 {
-    if (variant_steps(variant) > 1 && ncell >= ((size_t)1 << (ctx->state_aos ? 28 : 30)))
+    if (variant_steps(variant) > 1 && ncell >= ((size_t)1 << 28))
         return fail(SAM2D_ERR_PARAM, "grid of %zu cells too large for a multi-step variant "
                     "(32-bit byte offsets); choose a single-step variant", ncell);
     return SAM2D_OK;
@@ -373,8 +373,7 @@ int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int ngroups, int nshot
                 if (rec[j] >= 0) mask |= 1 << j;
             }
             HIP_TRY(launch_stepn(variant, desc, nshots, ntiles, ctx->grid_cap, kza, n, cur, cur ^ 1,
-                                 mask, band_w, pend, pend_state, max_ng, order, ctx->state_aos,
-                                 ctx->stream));
+                                 mask, band_w, pend, pend_state, max_ng, order, ctx->stream));
             for (int j = 0; j < BAND_SLICES; j++) pend[j] = rec[j];
             pend_state = rec_after(ta, n + T - 1);
             cur ^= 1;
@@ -384,28 +383,27 @@ int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int ngroups, int nshot
             // Fewer than T steps left: flush, then single steps (each kernel
             // gathers the step before it).
             HIP_TRY(launch_gather(desc, nshots, max_ng, band_w ^ 1, pend, cur, pend_state,
-                                  ctx->state_aos, ctx->stream));
+                                  ctx->stream));
             for (int j = 0; j < BAND_SLICES; j++) pend[j] = -1;
             for (bool first = true; n < n1; n++, first = false) {
                 HIP_TRY(launch_step(single, desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
-                                    first ? -1 : rec_slot_for_kernel(ta, n), ctx->state_aos,
-                                    ctx->stream));
+                                    first ? -1 : rec_slot_for_kernel(ta, n), ctx->stream));
                 cur ^= 1;
             }
             pend_state = rec_after(ta, n1 - 1);
         }
         HIP_TRY(launch_gather(desc, nshots, max_ng, band_w ^ 1, pend, cur, pend_state,
-                              ctx->state_aos, ctx->stream));
+                              ctx->stream));
     } else {
         for (; n < n1; n++) {
             HIP_TRY(launch_step(variant, desc, nshots, ntiles, ctx->grid_cap, n, cur, cur ^ 1,
-                                rec_slot_for_kernel(ta, n), ctx->state_aos, ctx->stream));
+                                rec_slot_for_kernel(ta, n), ctx->stream));
             cur ^= 1;
         }
         if (n1 == ta.fd_nt) {
             const int slot = rec_slot_for_kernel(ta, n1);
             if (slot >= 0 && max_ng > 0)
-                HIP_TRY(launch_record(desc, nshots, max_ng, cur, slot, ctx->state_aos, ctx->stream));
+                HIP_TRY(launch_record(desc, nshots, max_ng, cur, slot, ctx->stream));
         }
     }
     if (end_slot) *end_slot = cur;
@@ -423,7 +421,7 @@ int run_time_loop(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int max_n
     bool used_graph = false;
     if (ctx->use_graph) {
         GraphKey key{nshots, ntiles, variant, max_ng, ctx->grid_cap, ctx->tile_order,
-                     tile_chunk(ctx, variant), ctx->state_aos};
+                     tile_chunk(ctx, variant)};
         hipGraphExec_t exec = nullptr;
         auto it = ctx->graphs.find(key);
         if (it != ctx->graphs.end()) exec = it->second;
@@ -520,10 +518,8 @@ static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, flo
                                    hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             const int nnz = one[0].dom->host.nnz;
-            if (ctx->state_aos)      // cells 0 .. nnz-1, four fields each
-                std::fill(keep.begin(), keep.begin() + 4 * (size_t)nnz, 0.0f);
-            else
-                for (int f = 0; f < 4; f++) std::fill(&keep[f * N], &keep[f * N] + nnz, 0.0f);
+            // cells 0 .. nnz-1, four fields each
+            std::fill(keep.begin(), keep.begin() + 4 * (size_t)nnz, 0.0f);
         }
         int ntiles = 0;
         ctx->quirk_valid = false;

@@ -39,25 +39,21 @@ struct ShotDev {
     float dtdx, dtdz;      // fd_dt/dx, fd_dt/dz (float products, as the reference forms them)
     const float *prof_x;   // [4][nnx]: damp2a, damp2a1, damp2b, damp2b1
     const float *prof_z;   // [4][nnz]: damp1a, damp1a1, damp1b, damp1b1
-    float *state;          // slots of 4 fields (px, pz, vx, vz), each ncell floats:
-    size_t ncell;          //   field f of slot s at state + (4*s + f)*ncell
+    float *state;          // state slots, cell-interleaved: px, pz, vx, vz of cell i
+    size_t ncell;          //   of slot s at state + 4*(s*ncell + i)
+    float *ring;           // recomputation only: forward ring of triplets, slot m
+                           //   (segment step m) at ring + 3*m*ncell, cell i holding
+                           //   p after step m and vx, vz before it at [3i .. 3i+2]
     const int *gcell;      // [ng] receiver cells (ix*nnz + iz)
     float *seis;           // [ng][real_nt]
     float *band;           // [2][BAND_SLICES][band_h][nnx]: pressure of the receiver
                            // rows after the intermediate steps of a launch
 };
 
-__host__ __device__ inline float *slot_field(const ShotDev &d, int slot, int f)
-{
-    return d.state + ((size_t)slot * 4 + f) * d.ncell;
-}
-// The same slot in the cell-interleaved layout of the forward-modelling calls:
-// px, pz, vx, vz of cell i at [4i .. 4i+3].  Per column a wave then reads
-// and writes one contiguous kilobyte of state instead of four 256-byte runs
-// ncell*4 bytes apart, which the memory system streams about 18 % faster for
-// the multi-step kernels' tile pattern on MI355X.  The gradient's forward sweep
-// and checkpoints use it too; its recomputation ring keeps one array per field
-// (the adjoint reads single fields of the forward states).
+// State slot s: px, pz, vx, vz of cell i at [4i .. 4i+3].  Per column a wave
+// reads and writes one contiguous kilobyte of state instead of four 256-byte
+// runs ncell*4 bytes apart, which the memory system streams about 18 % faster
+// for the kernels' tile pattern on MI355X.
 __host__ __device__ inline float *slot_cells(const ShotDev &d, int slot)
 {
     return d.state + (size_t)slot * 4 * d.ncell;
@@ -73,15 +69,15 @@ struct AdjDev {
     int band_h, rrow_lo, rrow_hi;   // receiver rows (rband layout)
     float dtdx, dtdz;      // fd_dt/dx, fd_dt/dz as the coefficients use them
     const float *srcg;     // [nts] d(source increment)/d(kappa_s) = fd_dt*w[n]/(dx*dz)
-    const float *KX, *KZ, *BX, *BZ;
+    const float *coef;     // KX, BX, BZ (and KZ unless it equals KX) of cell i at
+                           //   [3i .. 3i+2] ([4i .. 4i+3] with KZ last)
     const float *prof_x, *prof_z;
-    const float *fwd;      // forward ring: slots of 4 fields (px, pz, vx, vz)
+    const float *ring;     // forward ring of triplets (ShotDev::ring)
     float *lam;            // adjoint state: 2 slots of 4*ncell floats, cell-interleaved
                            // (L_px, L_pz, L_vx, L_vz of cell i at 4i .. 4i+3)
     size_t ncell;
-    float *gk;             // dJ/dkappa      (padded grid)
-    float *gbx;            // dJ/dbu2        (x-staggered buoyancy)
-    float *gbz;            // dJ/dbu1        (z-staggered buoyancy)
+    float *acc;            // dJ/dkappa, dJ/dbu2 (x-staggered buoyancy), dJ/dbu1
+                           //   (z-staggered) of cell i at [3i .. 3i+2]
     const int *rcell;      // [ncells] distinct receiver cells (ix*nnz + iz)
     const int *rptr;       // [ncells+1] receivers of each cell (CSR)
     const int *rid;        // [ng] receiver index of each entry, ascending per cell
@@ -116,23 +112,23 @@ inline int per_xcd_for(int ntiles, TileOrder &order)
     return (ntiles + 7) / 8;
 }
 // One backward step n of every staged shot: adjoint state lam_in -> lam_out,
-// forward states S(n-1) in ring slot fwd_prev and S(n) in fwd_cur; rec_k >= 0
-// injects residual sample rec_k of the shots that carry an rband.
-// kz_alias: every staged shot's KZ is its KX.  psum bit 0: fwd_cur is a p-sum
-// slot (launch_stepn_ring).
+// forward triplets of step n in ring slot fslot; rec_k >= 0 injects residual
+// sample rec_k of the shots that carry an rband.  kz_alias: every staged
+// shot's KZ is its KX (three-float coefficient cells).
 hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
-                               bool kz_alias, int step, int lam_in, int lam_out, int fwd_prev,
-                               int fwd_cur, int rec_k, int psum, TileOrder order,
-                               hipStream_t stream);
+                               bool kz_alias, int step, int lam_in, int lam_out, int fslot,
+                               int rec_k, TileOrder order, hipStream_t stream);
 // Backward steps n and n-1 in one pass (all shots must carry an rband or no
-// receivers): forward vx, vz after step n-2 in ring slot fa, after n-1 in fb;
-// px, pz after n-1 in fb, after n in fc; residual samples rec1 (step n) and
-// rec2 (step n-1).  psum bit 0: fc is a p-sum slot, bit 1: fb is.  Tiles of
+// receivers): forward triplets of step n in ring slot fslot, of step n-1 in
+// fslot - 1; residual samples rec1 (step n) and rec2 (step n-1).  Tiles of
 // OWN_Z2-row strips.
 hipError_t launch_adjoint_step2(int variant, const AdjDev *d_adj, int nshots, int ntiles,
-                                bool kz_alias, int step, int lam_in, int lam_out, int fa, int fb,
-                                int fc, int rec1, int rec2, int psum, TileOrder order,
-                                hipStream_t stream);
+                                bool kz_alias, int step, int lam_in, int lam_out, int fslot,
+                                int rec1, int rec2, TileOrder order, hipStream_t stream);
+// The adjoint's coefficient cells of one domain of n cells: KX, BX, BZ at
+// [3i .. 3i+2] when KZ is KX (kz == NULL), else KX, BX, BZ, KZ at [4i .. 4i+3].
+hipError_t launch_pack_coef(const float *kx, const float *kz, const float *bx, const float *bz,
+                            size_t n, float *out, hipStream_t stream);
 // Dense residual band [real_nt][band_cells]: element (k, i) is the sum, in
 // rid order, of res[rid[e] * real_nt + k] over e in [rptr[u], rptr[u+1]) with
 // u = bmap[i], starting from -0; -0 where bmap[i] < 0.
@@ -158,42 +154,42 @@ int variant_single_step(int v);
 int variant_index(int waves, int chunk, int pf);
 
 // T = variant_steps(variant) time steps (step .. step+T-1) of every staged
-// shot, in_slot -> out_slot.  band_mask bit j keeps the pressure after step
-// step+j (j < T-1) of the receiver rows in band slice j of parity band_w.
-// Before that, the launch gathers the previous launch's samples:
+// shot, state slot in_slot -> out_slot.  band_mask bit j keeps the pressure
+// after step step+j (j < T-1) of the receiver rows in band slice j of parity
+// band_w.  Before that, the launch gathers the previous launch's samples:
 // band_samples[j] from slice j of parity band_w ^ 1 and state_sample from the
 // in_slot state (each skipped when < 0).  kz_alias: every staged shot's KZ is
-// its KX (dt/dz == dt/dx).  aos: the state slots are cell-interleaved
-// (slot_cells), else one array per field (slot_field).
+// its KX (dt/dz == dt/dx).
 hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
                         bool kz_alias, int step, int in_slot, int out_slot, int band_mask,
                         int band_w, const int *band_samples, int state_sample, int max_ng,
-                        TileOrder order, bool aos, hipStream_t stream);
-// The recomputation form: the states after all three steps of a three-step
-// variant are stored, into slots in_slot+1 .. in_slot+3 (no receivers).  The
-// intermediate slots in_slot+1 and in_slot+2 hold p = px + pz in the px field
-// and no pz (a "p-sum" slot); in_slot+3 is a full state.
+                        TileOrder order, hipStream_t stream);
+// The recomputation form of a three-step variant: state slot in_slot ->
+// in_slot ^ 1, and for each of the three steps step+t the triplet (p after
+// it, vx and vz before it) of every owned cell into ring slot ring_slot + t
+// (no receivers).
 bool ring_supported(int variant);
 hipError_t launch_stepn_ring(int variant, const ShotDev *d_shots, int nshots, int ntiles,
-                             int grid_cap, bool kz_alias, int step, int in_slot, TileOrder order,
-                             hipStream_t stream);
+                             int grid_cap, bool kz_alias, int step, int in_slot, int ring_slot,
+                             TileOrder order, hipStream_t stream);
 // The deferred gathers as a launch of their own (after the last multi-step
 // launch): band_samples[j] from slice j of parity band_r, the state slot.
 hipError_t launch_gather(const ShotDev *d_shots, int nshots, int max_ng, int band_r,
-                         const int *band_samples, int state_slot, int state_sample, bool aos,
+                         const int *band_samples, int state_slot, int state_sample,
                          hipStream_t stream);
 
 // One time step of every staged shot: reads state slot in_slot, writes
 // out_slot; rec_sample >= 0 gathers the receivers of the in_slot state.
-// aos as for launch_stepn.
 hipError_t launch_step(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
-                       int step, int in_slot, int out_slot, int rec_sample, bool aos,
-                       hipStream_t stream);
+                       int step, int in_slot, int out_slot, int rec_sample, hipStream_t stream);
+// The recomputation form of one step: in_slot -> out_slot and the step's
+// triplets into ring slot ring_slot; the single-step variant's tile shape
+// (stores without the non-temporal hint), no receivers.
+hipError_t launch_step_ring(int variant, const ShotDev *d_shots, int nshots, int ntiles,
+                            int grid_cap, int step, int in_slot, int out_slot, int ring_slot,
+                            hipStream_t stream);
 hipError_t launch_record(const ShotDev *d_shots, int nshots, int max_ng, int slot, int sample,
-                         bool aos, hipStream_t stream);
+                         hipStream_t stream);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t stream);
-// n cells of cell-interleaved state at src rearranged into one array per
-// field at dst (4 n floats each; src and dst do not overlap).
-hipError_t launch_cells_to_fields(const float *src, float *dst, size_t n, hipStream_t stream);
 
 }  // namespace sam2d

@@ -44,12 +44,14 @@ struct DeviceDomain {
     DomainHost host;
     float *KX = nullptr, *KZ = nullptr, *BX = nullptr, *BZ = nullptr, *B = nullptr;
     float *prof_x = nullptr, *prof_z = nullptr;
+    float *ADJ = nullptr;          // the adjoint's coefficient cells (launch_pack_coef), built
+                                   // by the first gradient on this domain
     ~DeviceDomain();
 };
 
 // Device state of one resident shot slot (grown on demand, reused).
 struct ShotBuffers {
-    float *state = nullptr;        // nslots x 4 fields x cells
+    float *state = nullptr;        // nslots x cells x 4 fields (cell-interleaved)
     float *amp = nullptr;
     int *gcell = nullptr;
     float *seis = nullptr;
@@ -75,12 +77,11 @@ struct ShotPlan {
 
 struct GraphKey {
     int nshots, ntiles, variant, max_ng, grid_cap, tile_order, chunk;
-    bool aos;
     bool operator<(const GraphKey &o) const
     {
-        return std::tie(nshots, ntiles, variant, max_ng, grid_cap, tile_order, chunk, aos) <
+        return std::tie(nshots, ntiles, variant, max_ng, grid_cap, tile_order, chunk) <
                std::tie(o.nshots, o.ntiles, o.variant, o.max_ng, o.grid_cap, o.tile_order,
-                        o.chunk, o.aos);
+                        o.chunk);
     }
 };
 
@@ -122,8 +123,6 @@ struct sam2d_ctx {
     int tile_order = 0;                    // chunk-column stride of multi-step kernels (tuned)
     int adjoint_tile_order = 7;            // chunk-column stride of the adjoint kernels
     int chunk = 0;                         // march length of multi-step kernels, 0 = variant's
-    bool state_aos = true;                 // cell-interleaved state (forward calls); the
-                                           // gradient runs with one array per field
     std::vector<double> tune_ms;           // per variant, from the autotuner
     sam2d_stats stats = sam2d_stats();
     sam2d::AdjointState *adj = nullptr;

@@ -38,6 +38,20 @@ __device__ __forceinline__ void dplus_coef(int i, int n, float &c0, float &c1)
 typedef float fquad __attribute__((ext_vector_type(4)));
 template <class T>
 __device__ __forceinline__ SAM2D_GLOBAL T *gp(T *p) { return (SAM2D_GLOBAL T *)p; }
+// Three floats of a cell (the forward ring's triplets, the adjoint's
+// accumulators and coefficients), 4-byte aligned: one dwordx3 access per lane.
+typedef float ftri __attribute__((ext_vector_type(3), aligned(4)));
+__device__ __forceinline__ ftri ld3(const SAM2D_GLOBAL char *__restrict__ base, unsigned off)
+{
+    return *(const SAM2D_GLOBAL ftri *__restrict__)(base + off);
+}
+__device__ __forceinline__ void st3(SAM2D_GLOBAL char *__restrict__ base, unsigned off, float a,
+                                    float b, float c)
+{
+    ftri v;
+    v.x = a; v.y = b; v.z = c;
+    *(SAM2D_GLOBAL ftri *__restrict__)(base + off) = v;
+}
 // Read-only tables indexed by a wave-uniform column (the x-profiles of the
 // sponge): in the constant address space the compiler reads them with scalar
 // loads, off the vector memory path that streams the state.
@@ -119,16 +133,13 @@ __device__ __forceinline__ SAM2D_GLOBAL float *band_slice(const ShotDev &d, int 
 // after cpu_transpose, :370-381): samples bs.x .. bs.w from band slices 0..3
 // of parity band_r, sample ss as px + pz of the state slot (each skipped when
 // < 0).
-template <bool AOS>
 __device__ __forceinline__ void gather_deferred(const ShotDev *sh, int nshots, int band_r, int4 bs,
                                                 int state_slot, int ss, int gtid, int gstride)
 {
     const int smp[BAND_SLICES] = {bs.x, bs.y, bs.z, bs.w};
     for (int s = 0; s < nshots; ++s) {
         const ShotDev &d = sh[s];
-        const SAM2D_GLOBAL float *px = gp(AOS ? slot_cells(d, state_slot) : slot_field(d, state_slot, 0));
-        const SAM2D_GLOBAL float *pz = gp(AOS ? slot_cells(d, state_slot) + 1 : slot_field(d, state_slot, 1));
-        const int fs = AOS ? 4 : 1;                   // floats between cells
+        const SAM2D_GLOBAL float *st = gp(slot_cells(d, state_slot));
         const SAM2D_GLOBAL int *gc = gp(d.gcell);
         SAM2D_GLOBAL float *seis = gp(d.seis);
         for (int g = gtid; g < d.ng; g += gstride) {
@@ -139,7 +150,7 @@ __device__ __forceinline__ void gather_deferred(const ShotDev *sh, int nshots, i
             #pragma unroll
             for (int j = 0; j < BAND_SLICES; j++)
                 if (smp[j] >= 0) tr[smp[j]] = band_slice(d, band_r, j)[at];
-            if (ss >= 0) tr[ss] = px[(size_t)fs * c] + pz[(size_t)fs * c];
+            if (ss >= 0) tr[ss] = st[4 * (size_t)c] + st[4 * (size_t)c + 1];
         }
     }
 }

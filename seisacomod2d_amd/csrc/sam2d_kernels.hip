@@ -9,8 +9,9 @@
 //   v-phase  Dx+ p,  Dz+ p,  vx/vz sponge update
 // plus the receiver gather of the previous step (cpu_record, :24-31).
 //
-// Layout: every field is column-major with z fastest, cell (ix, iz) at
-// ix*nnz + iz (the reference's id = iz + ix*nnz).  A wavefront owns a strip
+// Layout: cells are column-major with z fastest, cell (ix, iz) at
+// ix*nnz + iz (the reference's id = iz + ix*nnz), the state cell-interleaved
+// (px, pz, vx, vz of a cell together).  A wavefront owns a strip
 // of OWN_Z rows and carries a HALO_T-row halo above and HALO_B rows below in
 // the same 64 lanes, so z-neighbours are lane shuffles.  It marches along x
 // over a chunk of columns keeping a sliding register window of vx, vz and the
@@ -29,45 +30,33 @@
 
 namespace sam2d {
 
-// Receiver gather of p = px + pz from the given state buffers
-// (cpu_record, cpu_modelling_kernels.cpp:24-31, trace-major output
-// as after cpu_transpose, :370-381).
-template <bool AOS>
+// Receiver gather of p = px + pz from a state slot (cpu_record,
+// cpu_modelling_kernels.cpp:24-31, trace-major output as after
+// cpu_transpose, :370-381).
 __device__ __forceinline__ void gather_receivers(const ShotDev *sh, int nshots, int in_slot,
                                                  int slot, int gtid, int gstride)
 {
     for (int s = 0; s < nshots; ++s) {
         const ShotDev &d = sh[s];
-        const SAM2D_GLOBAL float *px = gp(AOS ? slot_cells(d, in_slot) : slot_field(d, in_slot, 0));
-        const SAM2D_GLOBAL float *pz = gp(AOS ? slot_cells(d, in_slot) + 1 : slot_field(d, in_slot, 1));
-        const size_t fs = AOS ? 4 : 1;               // floats between cells
+        const SAM2D_GLOBAL float *st = gp(slot_cells(d, in_slot));
         const SAM2D_GLOBAL int *gc = gp(d.gcell);
         SAM2D_GLOBAL float *seis = gp(d.seis);
         for (int g = gtid; g < d.ng; g += gstride) {
-            const size_t c = fs * (size_t)gc[g];
-            seis[(size_t)g * d.real_nt + slot] = px[c] + pz[c];
+            const size_t c = 4 * (size_t)gc[g];
+            seis[(size_t)g * d.real_nt + slot] = st[c] + st[c + 1];
         }
     }
 }
 
 // Operands of one column that the march loads (one row per lane).
 struct ColOps {
-    float vx_next;              // vx at column c+1
-    float vz, px, pz, kx, kz;   // at column c
+    fquad s;                    // state at column c+1, kept whole until consumed
+    float kx, kz;               // at column c
     float dax, skx;             // x-profiles at column c (uniform)
-    float bx, bz, vz_j;         // BX, BZ and vz at column j = c-2
+    float bx, bz;               // BX, BZ at column j = c-2
     float dbx, sbx;             // x-profiles at column j (uniform)
-    fquad s;                    // cell-interleaved state only: state at column c+1
 };
 
-template <bool NT>
-__device__ __forceinline__ void st(SAM2D_GLOBAL float *p, unsigned i, float v)
-{
-    if (NT) __builtin_nontemporal_store(v, p + i);
-    else p[i] = v;
-}
-
-// Store of a cell's four state fields (cell-interleaved layout).
 template <bool NT>
 __device__ __forceinline__ void st4(SAM2D_GLOBAL fquad *p, unsigned i, float a, float b, float c,
                                     float e)
@@ -83,7 +72,9 @@ __device__ __forceinline__ void st4(SAM2D_GLOBAL fquad *p, unsigned i, float a, 
 // sample rec_slot.
 //
 // WAVES: z-strips per workgroup.  PF: columns of operands in flight (2 or 4).
-// NTS: non-temporal hint on the state stores.
+// NTS: non-temporal hint on the state stores.  RING (the gradient's
+// recomputation): the triplet (p after the step, vx and vz before it) of
+// every owned cell also goes to ring slot ring_slot.
 //
 // The march along x is unrolled by four columns so that the sliding windows
 // (vx at c-2..c+1, new p at c-3..c) rotate by compile-time index and the
@@ -95,21 +86,20 @@ __device__ __forceinline__ void st4(SAM2D_GLOBAL fquad *p, unsigned i, float a, 
 // finite in-grid value, and every such value meets a zero stencil coefficient
 // (the reference's edge rules) before it could reach an owned cell.
 //
-// AOS: cell-interleaved state slots (slot_cells): one 16-byte load per column
-// brings the state of column c+1, whose px, pz, vz wait in windows; the
-// v-phase at j stores all four fields of column j, px and pz kept from its
-// p-phase two columns earlier.
-template <int WAVES, int PF, bool NTS, bool AOS>
+// One 16-byte load per column brings the state of column c+1, whose px, pz,
+// vz wait in windows; the v-phase at j stores all four fields of column j,
+// px and pz kept from its p-phase two columns earlier.
+template <int WAVES, int PF, bool NTS, bool RING>
 __global__ __launch_bounds__(WAVES * 64)
 void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
-            int ntiles, int step, int in_slot, int out_slot, int rec_slot)
+            int ntiles, int step, int in_slot, int out_slot, int rec_slot, int ring_slot)
 {
     static_assert(PF == 1 || PF == 2 || PF == 4, "PF must divide the unroll of 4");
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
 
     if (rec_slot >= 0)
-        gather_receivers<AOS>(shots, nshots, in_slot, rec_slot,
+        gather_receivers(shots, nshots, in_slot, rec_slot,
                          blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 
     for (int v = blockIdx.x; v < vtotal; v += gridDim.x) {
@@ -130,16 +120,10 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         const bool rown = (lane >= HALO_T) && (lane < HALO_T + OWN_Z) && (r < nnz);
         const unsigned rc = (unsigned)min(max(r, 0), nnz - 1);
 
-        const SAM2D_GLOBAL float *__restrict__ pxi = gp(slot_field(d, in_slot, 0));
-        const SAM2D_GLOBAL float *__restrict__ pzi = gp(slot_field(d, in_slot, 1));
-        const SAM2D_GLOBAL float *__restrict__ vxi = gp(slot_field(d, in_slot, 2));
-        const SAM2D_GLOBAL float *__restrict__ vzi = gp(slot_field(d, in_slot, 3));
-        SAM2D_GLOBAL float *__restrict__ pxo = gp(slot_field(d, out_slot, 0));
-        SAM2D_GLOBAL float *__restrict__ pzo = gp(slot_field(d, out_slot, 1));
-        SAM2D_GLOBAL float *__restrict__ vxo = gp(slot_field(d, out_slot, 2));
-        SAM2D_GLOBAL float *__restrict__ vzo = gp(slot_field(d, out_slot, 3));
         const SAM2D_GLOBAL fquad *__restrict__ si = (const SAM2D_GLOBAL fquad *)gp(slot_cells(d, in_slot));
         SAM2D_GLOBAL fquad *__restrict__ so = (SAM2D_GLOBAL fquad *)gp(slot_cells(d, out_slot));
+        SAM2D_GLOBAL char *__restrict__ ro =
+            RING ? (SAM2D_GLOBAL char *)gp(d.ring + (size_t)ring_slot * 3 * d.ncell) : nullptr;
         const SAM2D_GLOBAL float *__restrict__ KX = gp(d.KX);
         const SAM2D_GLOBAL float *__restrict__ KZ = gp(d.KZ);
         const SAM2D_GLOBAL float *__restrict__ BX = gp(d.BX);
@@ -164,13 +148,7 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         auto load_col = [&](int c) -> ColOps {
             ColOps o;
             const unsigned b1 = colbase(c + 1), b0 = colbase(c), bj = colbase(c - 2);
-            if constexpr (AOS) {
-                o.s = si[b1 + rc];
-            } else {
-                o.vx_next = vxi[b1 + rc];
-                o.vz = vzi[b0 + rc]; o.px = pxi[b0 + rc]; o.pz = pzi[b0 + rc];
-                o.vz_j = vzi[bj + rc];
-            }
+            o.s = si[b1 + rc];
             o.kx = KX[b0 + rc]; o.kz = KZ[b0 + rc];
             o.bx = BX[bj + rc]; o.bz = BZ[bj + rc];
             const int cc = min(max(c, 0), nnx - 1), jc = min(max(c - 2, 0), nnx - 1);
@@ -180,21 +158,15 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         };
 
         // Windows, indexed modulo 4 relative to the unrolled block's base
-        // column c: vxw[k & 3] = vx at c-2+k; pw[k & 3] = new p at c-3+k.
-        // AOS also: pxw, pzw, vzw (old px, pz, vz, like vxw) and pxnw, pznw
-        // (new px, pz, like pw).
+        // column c: vxw, pxw, pzw, vzw[k & 3] = old vx, px, pz, vz at c-2+k;
+        // pw, pxnw, pznw[k & 3] = new p, px, pz at c-3+k.
         float vxw[4], pw[4], pxw[4], pzw[4], vzw[4], pxnw[4], pznw[4];
         #pragma unroll
         for (int k = 0; k < 4; k++) vxw[k] = pw[k] = pxw[k] = pzw[k] = vzw[k] = pxnw[k] = pznw[k] = 0.0f;
         #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const unsigned b = colbase(xa - 3 + k) + rc;
-            if constexpr (AOS) {
-                const fquad sv = si[b];
-                pxw[k] = sv.x; pzw[k] = sv.y; vxw[k] = sv.z; vzw[k] = sv.w;
-            } else {
-                vxw[k] = vxi[b];
-            }
+            const fquad sv = si[colbase(xa - 3 + k) + rc];
+            pxw[k] = sv.x; pzw[k] = sv.y; vxw[k] = sv.z; vzw[k] = sv.w;
         }
 
         ColOps q[PF];
@@ -210,38 +182,30 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
                 if (c > xb + 1) break;
                 ColOps &cur = q[u % PF];
                 // ---- p-phase at column c (cpu_pml_p, :335-351; source, :135-140)
-                if constexpr (AOS) {
-                    pxw[(u + 3) & 3] = cur.s.x;
-                    pzw[(u + 3) & 3] = cur.s.y;
-                    vxw[(u + 3) & 3] = cur.s.z;
-                    vzw[(u + 3) & 3] = cur.s.w;
-                } else {
-                    vxw[(u + 3) & 3] = cur.vx_next;
-                }
+                pxw[(u + 3) & 3] = cur.s.x;
+                pzw[(u + 3) & 3] = cur.s.y;
+                vxw[(u + 3) & 3] = cur.s.z;
+                vzw[(u + 3) & 3] = cur.s.w;
                 const float vx_m2 = vxw[u & 3], vx_m1 = vxw[(u + 1) & 3];
                 const float vx_0 = vxw[(u + 2) & 3], vx_p1 = vxw[(u + 3) & 3];
                 float xm0, xm1;
                 dminus_coef(c, nnx, xm0, xm1);
                 const float dvx = fmaf(xm0, vx_0 - vx_m1, xm1 * (vx_p1 - vx_m2));
-                const float vz_0 = AOS ? vzw[(u + 2) & 3] : cur.vz;
+                const float vz_0 = vzw[(u + 2) & 3];
                 const float vz_u2 = shfl_up(vz_0, 2), vz_u1 = shfl_up(vz_0, 1);
                 const float vz_d1 = shfl_dn(vz_0, 1);
                 const float dvz = fmaf(zm0, vz_0 - vz_u1, zm1 * (vz_d1 - vz_u2));
-                const float px_old = AOS ? pxw[(u + 2) & 3] : cur.px;
-                const float pz_old = AOS ? pzw[(u + 2) & 3] : cur.pz;
+                const float px_old = pxw[(u + 2) & 3];
+                const float pz_old = pzw[(u + 2) & 3];
                 float pxn = fmaf(cur.kx * cur.skx, dvx, cur.dax * px_old);
                 const float pzn = fmaf(cur.kz * skz, dvz, daz * pz_old);
                 if (src_row && c == src_x) pxn += amp;
                 const float pn = pxn + pzn;
                 pw[(u + 3) & 3] = pn;
-                if (AOS) {
-                    pxnw[(u + 3) & 3] = pxn;          // stored with vx, vz at j
-                    pznw[(u + 3) & 3] = pzn;
-                } else if (rown && c >= xa && c < xb) {
-                    const unsigned id = (unsigned)c * (unsigned)nnz + (unsigned)r;
-                    st<NTS>(pxo, id, pxn);
-                    st<NTS>(pzo, id, pzn);
-                }
+                pxnw[(u + 3) & 3] = pxn;              // stored with vx, vz at j
+                pznw[(u + 3) & 3] = pzn;
+                if (RING && rown && c >= xa && c < xb)
+                    st3(ro, ((unsigned)c * (unsigned)nnz + (unsigned)r) * 12u, pn, vx_0, vz_0);
 
                 // ---- v-phase at column j = c - 2 (cpu_pml_v, :353-368)
                 const int j = c - 2;
@@ -254,16 +218,11 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
                     const float q_d1 = shfl_dn(p_m2, 1), q_d2 = shfl_dn(p_m2, 2);
                     const float dpz = fmaf(zp0, q_d1 - p_m2, zp1 * (q_d2 - q_u1));
                     const float vxn = fmaf(cur.bx * cur.sbx, dpx, cur.dbx * vx_m2);
-                    const float vz_j = AOS ? vzw[u & 3] : cur.vz_j;
+                    const float vz_j = vzw[u & 3];
                     const float vzn = fmaf(cur.bz * sbz, dpz, dbz * vz_j);
                     if (rown) {
                         const unsigned id = (unsigned)j * (unsigned)nnz + (unsigned)r;
-                        if constexpr (AOS) {
-                            st4<NTS>(so, id, pxnw[(u + 1) & 3], pznw[(u + 1) & 3], vxn, vzn);
-                        } else {
-                            st<NTS>(vxo, id, vxn);
-                            st<NTS>(vzo, id, vzn);
-                        }
+                        st4<NTS>(so, id, pxnw[(u + 1) & 3], pznw[(u + 1) & 3], vxn, vzn);
                     }
                 }
                 cur = load_col(min(c + PF, xb + 1));
@@ -273,34 +232,10 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
 }
 
 // Receiver gather alone (the last step's samples).
-template <bool AOS>
 __global__ void k_record(const ShotDev *__restrict__ shots, int nshots, int in_slot, int slot)
 {
-    gather_receivers<AOS>(shots, nshots, in_slot, slot, blockIdx.x * blockDim.x + threadIdx.x,
-                          gridDim.x * blockDim.x);
-}
-
-// Cell-interleaved state (px, pz, vx, vz of cell i at 4i..4i+3) to one array
-// per field (field f of cell i at f*n + i).
-__global__ void k_cells_to_fields(const float4 *__restrict__ src, float *__restrict__ dst, size_t n)
-{
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const float4 v = src[i];
-        dst[i] = v.x;
-        dst[n + i] = v.y;
-        dst[2 * n + i] = v.z;
-        dst[3 * n + i] = v.w;
-    }
-}
-
-hipError_t launch_cells_to_fields(const float *src, float *dst, size_t n, hipStream_t stream)
-{
-    if (n == 0) return hipSuccess;
-    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_cells_to_fields, dim3((unsigned)blocks), dim3(256), 0, stream,
-                       (const float4 *)src, dst, n);
-    return hipGetLastError();
+    gather_receivers(shots, nshots, in_slot, slot, blockIdx.x * blockDim.x + threadIdx.x,
+                     gridDim.x * blockDim.x);
 }
 
 // Fill with a constant.  Used instead of hipMemsetAsync so that clearing is
@@ -407,28 +342,33 @@ void variant_name(int v, char *buf, int n)
     else snprintf(buf, n, "waves%d_chunk%d_pf%d%s", w, c, pf, nts ? "_nts" : "");
 }
 
-template <int W, int PF, bool NTS, bool AOS>
-static void launch_t(dim3 g, hipStream_t s, const ShotDev *d, int ns, int vt, int per, int nt,
-                     int step, int in, int out, int slot)
+struct StepArgs {
+    const ShotDev *d;
+    int ns, vt, per, nt, step, in, out, rec, ring;
+};
+
+template <int W, int PF, bool NTS, bool RING>
+static void launch_t(dim3 g, hipStream_t s, const StepArgs &a)
 {
-    hipLaunchKernelGGL((k_step<W, PF, NTS, AOS>), g, dim3(64 * W), 0, s, d, ns, vt, per, nt, step,
-                       in, out, slot);
+    hipLaunchKernelGGL((k_step<W, PF, NTS, RING>), g, dim3(64 * W), 0, s, a.d, a.ns, a.vt, a.per,
+                       a.nt, a.step, a.in, a.out, a.rec, a.ring);
 }
 
-template <int W, bool AOS>
-static void launch_w(int pf, int nts, dim3 g, hipStream_t s, const ShotDev *d, int ns, int vt,
-                     int per, int nt, int step, int in, int out, int slot)
+template <int W>
+static void launch_w(int pf, int nts, bool ring, dim3 g, hipStream_t s, const StepArgs &a)
 {
-    if (pf == 2 && !nts) launch_t<W, 2, false, AOS>(g, s, d, ns, vt, per, nt, step, in, out, slot);
-    else if (pf == 2) launch_t<W, 2, true, AOS>(g, s, d, ns, vt, per, nt, step, in, out, slot);
-    else if (!nts) launch_t<W, 4, false, AOS>(g, s, d, ns, vt, per, nt, step, in, out, slot);
-    else launch_t<W, 4, true, AOS>(g, s, d, ns, vt, per, nt, step, in, out, slot);
+    if (ring) {
+        if (pf == 2) launch_t<W, 2, false, true>(g, s, a);
+        else launch_t<W, 4, false, true>(g, s, a);
+    } else if (pf == 2 && !nts) launch_t<W, 2, false, false>(g, s, a);
+    else if (pf == 2) launch_t<W, 2, true, false>(g, s, a);
+    else if (!nts) launch_t<W, 4, false, false>(g, s, a);
+    else launch_t<W, 4, true, false>(g, s, a);
 }
 
-template <bool AOS>
 static hipError_t launch_step_l(int variant, const ShotDev *d_shots, int nshots, int ntiles,
                                 int grid_cap, int step, int in_slot, int out_slot, int rec_slot,
-                                hipStream_t stream)
+                                int ring_slot, bool ring, hipStream_t stream)
 {
     int waves, chunk, pf, nts;
     decode(variant, waves, chunk, pf, nts);
@@ -437,37 +377,39 @@ static hipError_t launch_step_l(int variant, const ShotDev *d_shots, int nshots,
     int grid = vtotal;
     if (grid_cap > 0 && grid > grid_cap) grid = (grid_cap + 7) / 8 * 8;
     const dim3 g(grid);
+    const StepArgs a{d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot,
+                     ring_slot};
     switch (waves) {
-    case 1: launch_w<1, AOS>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
-    case 2: launch_w<2, AOS>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
-    case 4: launch_w<4, AOS>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
-    case 8: launch_w<8, AOS>(pf, nts, g, stream, d_shots, nshots, vtotal, per, ntiles, step, in_slot, out_slot, rec_slot); break;
+    case 1: launch_w<1>(pf, nts, ring, g, stream, a); break;
+    case 2: launch_w<2>(pf, nts, ring, g, stream, a); break;
+    case 4: launch_w<4>(pf, nts, ring, g, stream, a); break;
+    case 8: launch_w<8>(pf, nts, ring, g, stream, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_step(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
-                       int step, int in_slot, int out_slot, int rec_slot, bool aos,
-                       hipStream_t stream)
+                       int step, int in_slot, int out_slot, int rec_slot, hipStream_t stream)
 {
-    return aos ? launch_step_l<true>(variant, d_shots, nshots, ntiles, grid_cap, step, in_slot,
-                                     out_slot, rec_slot, stream)
-               : launch_step_l<false>(variant, d_shots, nshots, ntiles, grid_cap, step, in_slot,
-                                      out_slot, rec_slot, stream);
+    return launch_step_l(variant, d_shots, nshots, ntiles, grid_cap, step, in_slot, out_slot,
+                         rec_slot, 0, false, stream);
+}
+
+hipError_t launch_step_ring(int variant, const ShotDev *d_shots, int nshots, int ntiles,
+                            int grid_cap, int step, int in_slot, int out_slot, int ring_slot,
+                            hipStream_t stream)
+{
+    return launch_step_l(variant, d_shots, nshots, ntiles, grid_cap, step, in_slot, out_slot, -1,
+                         ring_slot, true, stream);
 }
 
 hipError_t launch_record(const ShotDev *d_shots, int nshots, int max_ng, int in_slot, int slot,
-                         bool aos, hipStream_t stream)
+                         hipStream_t stream)
 {
     int blocks = (max_ng + 255) / 256;
     if (blocks < 1) blocks = 1;
-    if (aos)
-        hipLaunchKernelGGL(k_record<true>, dim3(blocks), dim3(256), 0, stream, d_shots, nshots,
-                           in_slot, slot);
-    else
-        hipLaunchKernelGGL(k_record<false>, dim3(blocks), dim3(256), 0, stream, d_shots, nshots,
-                           in_slot, slot);
+    hipLaunchKernelGGL(k_record, dim3(blocks), dim3(256), 0, stream, d_shots, nshots, in_slot, slot);
     return hipGetLastError();
 }
 

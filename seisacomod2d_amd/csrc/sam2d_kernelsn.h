@@ -1,0 +1,415 @@
+// sam2d_kernelsn.h -- the forward step advanced T time steps per pass
+// (temporal blocking of depth T = 2..5) for gfx950.
+//
+// The per-cell arithmetic is that of k_step (sam2d_kernels.hip), so every T
+// gives the same bits.  One launch carries the state from step n-1 to step
+// n+T-1; the states in between stay in registers, so a cell's four fields and
+// three coefficients cross HBM once per T updates (44 bytes per T
+// cell-updates, plus halos).
+//
+// A wavefront covers 64 rows: it owns 64 - 6T and carries a 3T-row halo above
+// and below, the dependency cone of T steps (each step reaches two rows one
+// way and one row the other in each phase).  It marches along x; at column c
+// step s (s = 0 .. T-1) runs its p-phase at column c - 3s and its v-phase at
+// column c - 3s - 2:
+//   p-phase s at cp = c-3s    needs vx(s-1) at cp-2..cp+1, vz, px, pz(s-1) at cp
+//   v-phase s at jv = c-3s-2  needs p(s) at jv-1..jv+2, vx, vz(s-1) at jv
+// Every intermediate quantity lives in a four-column window indexed by
+// compile-time position over a march unrolled by four; a coefficient needed
+// by a later step moves from one step's window to the next.
+//
+// Receivers: the pressure after each intermediate step is kept for the
+// receiver rows in a band slice; the state after the last step is stored.
+// The next launch gathers all T samples (gather_deferred).
+//
+// State layout: cell-interleaved (px, pz, vx, vz of a cell together): the
+// state of a column arrives as one 16-byte load per lane and leaves as one
+// 16-byte store, a contiguous kilobyte per wave, which streams about 18 %
+// faster on MI355X than four fields in separate arrays.
+//
+// Instruction economy:
+//  * z-neighbours move between lanes with DPP wavefront shifts;
+//  * memory is addressed as a uniform base plus a 32-bit per-lane byte offset,
+//    one offset per column shared by every field, and the prefetch slots are
+//    emptied through keep() so each next load lands in place;
+//  * groups of four columns where the x sponge profiles are exactly 1 and the
+//    stencil has its 4th-order coefficients take a fast path without profile
+//    loads or coefficient selects (multiplying by exactly 1 is exact, so both
+//    paths give the same bits); slow groups, the fast range and slow groups
+//    run as three loops in sequence.
+
+#pragma once
+#include <hip/hip_runtime.h>
+#include <type_traits>
+#include "sam2d_device.h"
+#include "sam2d_kcommon.h"
+#include "sam2d_round.h"
+
+namespace sam2d {
+
+struct ColN {
+    float kx, kz;               // at column c (kz unused when KZ aliases KX)
+    float b;                    // buoyancy 1/rho at column c-1
+    fquad s;                    // the state at column c+1, kept whole until the
+                                // column consumes it
+};
+
+// RING: the gradient's recomputation of a checkpoint segment.  For each step
+// t the p-phase also stores, for every owned cell of its column, the triplet
+// (p after step+t, vx and vz before it) into ring slot ring_slot + t: what
+// one backward step reads of the forward, 12 bytes per cell and step, in one
+// dwordx3 store per lane.  No receivers are recorded.
+// One 16-byte load per column and row brings px, pz, vx, vz of column c+1;
+// px, pz, vz wait a column (vz three) in windows, and the last step stores
+// all four fields of column jv at once, px and pz kept from its p-phase two
+// columns earlier.
+// Three-step kernels are held to three waves per SIMD (at most 168 VGPRs),
+// the occupancy their measured best shapes need.
+template <int T, int WAVES, int PF, bool KZA, bool RING>
+__global__ __launch_bounds__(WAVES * 64, T == 3 ? 3 : 1)
+void k_stepn(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_xcd, int ntiles,
+             int step, int in_slot, int out_slot, int band_mask, int band_w, int4 band_s,
+             int state_sample, int ngroups, int xorder, int ring_slot)
+{
+    static_assert(T >= 2 && T <= BAND_SLICES + 1, "steps per pass");
+    static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
+    constexpr int HALO = 3 * T;
+    constexpr int OWN = 64 - 6 * T;
+    using gcchar = const SAM2D_GLOBAL char;
+    using gchar = SAM2D_GLOBAL char;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+    if (band_s.x >= 0 || band_s.y >= 0 || band_s.z >= 0 || band_s.w >= 0 || state_sample >= 0)
+        gather_deferred(shots, nshots, band_w ^ 1, band_s, in_slot, state_sample,
+                        blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+
+    for (int v = blockIdx.x; v < vtotal; v += gridDim.x) {
+        const int tile = xcd_tile_ordered(v, per_xcd, ngroups, xorder);
+        if (tile >= ntiles) continue;
+        const int s = shot_of_tile(shots, nshots, tile);
+        const ShotDev &d = shots[s];
+        const int local = tile - d.tile_begin;
+        const int group = local % d.ngroups;
+        const int chunk = local / d.ngroups;
+        const int nnz = d.nnz, nnx = d.nnx;
+        const int zb = (group * WAVES + wv) * OWN;
+        if (zb >= nnz) continue;                      // wave-uniform
+        const int xa = chunk * d.chunk;
+        const int xb = min(xa + d.chunk, nnx);
+
+        const int r = zb - HALO + lane;
+        const bool rown = (lane >= HALO) && (lane < HALO + OWN) && (r < nnz);
+        const unsigned rc = (unsigned)min(max(r, 0), nnz - 1);
+        const unsigned rc4 = rc * 4u;
+
+        // Inputs, outputs and coefficients never overlap.
+        gcchar *__restrict__ si = (gcchar *)slot_cells(d, in_slot);
+        gchar *__restrict__ so = (gchar *)slot_cells(d, out_slot);
+        gcchar *__restrict__ KX = (gcchar *)d.KX;
+        gcchar *__restrict__ KZ = (gcchar *)d.KZ;
+        gcchar *__restrict__ Bf = (gcchar *)d.B;
+        const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);
+        const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
+        // Ring slots of the T steps (RING only).
+        gchar *__restrict__ ring[T];
+        #pragma unroll
+        for (int t = 0; t < T; t++)
+            ring[t] = RING ? (gchar *)(d.ring + (size_t)(ring_slot + t) * 3 * d.ncell) : nullptr;
+
+        auto ld = [](gcchar *__restrict__ base, unsigned off) -> float {
+            return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
+        };
+        // Cell-interleaved state: byte offset 4x that of one field.
+        auto ld4 = [](gcchar *__restrict__ base, unsigned off) -> fquad {
+            return *(const SAM2D_GLOBAL fquad *__restrict__)(base + off);
+        };
+        auto st4 = [](gchar *__restrict__ base, unsigned off, float a, float b, float c, float e) {
+            fquad v;
+            v.x = a; v.y = b; v.z = c; v.w = e;
+            *(SAM2D_GLOBAL fquad *__restrict__)(base + off) = v;
+        };
+        // Byte offset of this lane's row in column col (clamped to the grid).
+        auto coff = [&](int col) -> unsigned {
+            return (unsigned)min(max(col, 0), nnx - 1) * (unsigned)nnz * 4u + rc4;
+        };
+        auto clampc = [&](int col) -> int { return min(max(col, 0), nnx - 1); };
+
+        const float daz = prz[rc], skz = prz[nnz + rc];
+        const float dbz = prz[2 * nnz + rc], sbz = prz[3 * nnz + rc];
+        float zm0, zm1, zp0, zp1;
+        dminus_coef(r, nnz, zm0, zm1);
+        dplus_coef(r, nnz, zp0, zp1);
+        const bool src_row = (r == d.src_z);
+        // Source of step step+k: injected while step+k <= nts, the increment
+        // past the wavelet's end being 0.
+        float amp[T];
+        int srcx[T];
+        #pragma unroll
+        for (int k = 0; k < T; k++) {
+            amp[k] = (step + k < d.nts) ? gp(d.amp)[step + k] : 0.0f;
+            srcx[k] = (step + k <= d.nts) ? d.src_x : -1;
+        }
+
+        const bool strip_band = d.rrow_lo <= d.rrow_hi && d.rrow_lo <= zb + OWN - 1 &&
+                                d.rrow_hi >= zb;
+        // Band slices this launch writes (restrict: they overlap nothing else
+        // the march touches, so the descriptor is not read again).
+        SAM2D_GLOBAL float *__restrict__ bandp[T];
+        #pragma unroll
+        for (int t = 0; t < T; t++) bandp[t] = band_slice(d, band_w, t < T - 1 ? t : 0);
+        const bool band_lane = rown && r >= d.rrow_lo && r <= d.rrow_hi;
+        // Fast groups start at c0 in [fast_lo, fast_hi]: columns
+        // c0-3T+1 .. c0+3 inside the all-ones x-profile range and the
+        // 4th-order stencil range (c0 >= 3T, c0 <= nnx-5).
+        const int fast_lo = max(d.xin_lo + 3 * T - 1, 3 * T);
+        const int fast_hi = min(d.xin_hi - 3, nnx - 5);
+
+        auto load_col = [&](int c) -> ColN {
+            ColN o;
+            o.s = ld4(si, 4 * coff(c + 1));
+            const unsigned o0 = coff(c);
+            o.kx = ld(KX, o0);
+            o.kz = KZA ? 0.0f : ld(KZ, o0);
+            o.b = ld(Bf, coff(c - 1));
+            return o;
+        };
+
+        // Windows, [k & 3] relative to the unrolled base:
+        //   vxin          vx(n-1) at c-2 .. c+1
+        //   P, PX, PZ[s]  p-phase s results at c-3s-3 .. c-3s (and KXw, KZw)
+        //   VX, VZ[s]     v-phase s results at c-3s-5 .. c-3s-2 (and BXw, BZw)
+        //   pxin, pzin, vzin  px, pz, vz(n-1) at c-2 .. c+1
+        float vxin[4], pxin[4], pzin[4], vzin[4];
+        float P[T][4], PX[T][4], PZ[T][4], KXw[T][4], KZw[T][4];
+        float VX[T][4], VZ[T][4], BXw[T][4], BZw[T][4];
+        #pragma unroll
+        for (int k = 0; k < 4; k++) {
+            vxin[k] = pxin[k] = pzin[k] = vzin[k] = 0.0f;
+            #pragma unroll
+            for (int t = 0; t < T; t++) {
+                P[t][k] = PX[t][k] = PZ[t][k] = KXw[t][k] = KZw[t][k] = 0.0f;
+                VX[t][k] = VZ[t][k] = BXw[t][k] = BZw[t][k] = 0.0f;
+            }
+        }
+        // Columns c_first .. c_last: the last step stores jv = c - 3T + 1 in
+        // [xa, xb).  Prefetches past c_last re-read column c_last (a cache
+        // hit) and the march ends at c_last exactly.
+        const int c_first = xa - (3 * T - 2);
+        const int c_last = xb + 3 * T - 2;
+        #pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const fquad sv = ld4(si, 4 * coff(c_first - 2 + k));
+            pxin[k] = sv.x; pzin[k] = sv.y; vxin[k] = sv.z; vzin[k] = sv.w;
+        }
+
+        // Buoyancy at the v-phase column of the first march column,
+        // jv = c_first - 2.
+        float b_0 = ld(Bf, coff(c_first - 2));
+        const float dtdx = d.dtdx, dtdz = d.dtdz;
+
+        ColN q[PF];
+        #pragma unroll
+        for (int u = 0; u < PF; u++) q[u] = load_col(c_first + u);
+
+        auto column = [&](auto fast_tag, auto u_tag, int c0) {
+            constexpr bool FAST = decltype(fast_tag)::value;
+            constexpr int u = decltype(u_tag)::value;
+            constexpr int i0 = u & 3, i1 = (u + 1) & 3, i2 = (u + 2) & 3, i3 = (u + 3) & 3;
+            const int c = c0 + u;
+            ColN &cur = q[u % PF];
+            pxin[i3] = keep(cur.s.x);
+            pzin[i3] = keep(cur.s.y);
+            vxin[i3] = keep(cur.s.z);
+            vzin[i3] = keep(cur.s.w);
+
+            #pragma unroll
+            for (int t = 0; t < T; t++) {
+                // ---- p-phase of step t at column cp
+                const int cp = c - 3 * t;
+                {
+                    float m0 = SAM2D_A0, m1 = SAM2D_A1;
+                    if (!FAST) dminus_coef(cp, nnx, m0, m1);
+                    const float *vxw = t == 0 ? vxin : VX[t > 0 ? t - 1 : 0];
+                    const float dvx = fmaf(m0, vxw[i2] - vxw[i1], m1 * (vxw[i3] - vxw[i0]));
+                    const float vz_0 = t == 0 ? vzin[i2] : VZ[t > 0 ? t - 1 : 0][i2];
+                    const float vz_u1 = lane_up1(vz_0), vz_u2 = lane_up1(vz_u1);
+                    const float vz_d1 = lane_dn1(vz_0);
+                    const float dvz = fmaf(zm0, vz_0 - vz_u1, zm1 * (vz_d1 - vz_u2));
+                    float kx, kz, px_old, pz_old;
+                    if (t == 0) {
+                        kx = keep(cur.kx);
+                        kz = KZA ? kx : keep(cur.kz);
+                        px_old = pxin[i2];
+                        pz_old = pzin[i2];
+                    } else {
+                        kx = KXw[t - 1][i0];
+                        kz = KZA ? kx : KZw[t - 1][i0];
+                        px_old = PX[t - 1][i0];
+                        pz_old = PZ[t - 1][i0];
+                    }
+                    float pxn;
+                    if (FAST) {
+                        pxn = fmaf(kx, dvx, px_old);
+                    } else {
+                        const int cc = clampc(cp);
+                        pxn = fmaf(kx * prx[nnx + cc], dvx, prx[cc] * px_old);
+                    }
+                    if (src_row && cp == srcx[t]) pxn += amp[t];
+                    const float pzn = fmaf(kz * skz, dvz, daz * pz_old);
+                    P[t][i3] = pxn + pzn;
+                    PX[t][i3] = pxn;                      // the last step's are stored
+                    PZ[t][i3] = pzn;                      // with vx, vz at jv
+                    if (t < T - 1) {
+                        KXw[t][i3] = kx;
+                        if (!KZA) KZw[t][i3] = kz;
+                        if (((band_mask >> t) & 1) && strip_band && cp >= xa && cp < xb) {
+                            if (band_lane) bandp[t][(size_t)r * nnx + cp] = P[t][i3];
+                        }
+                    }
+                    if (RING && rown && cp >= xa && cp < xb)
+                        st3(ring[t], ((unsigned)cp * (unsigned)nnz + rc) * 12u, P[t][i3],
+                            vxw[i2], vz_0);
+                }
+                // ---- v-phase of step t at column jv
+                const int jv = cp - 2;
+                {
+                    float q0 = SAM2D_A0, q1 = SAM2D_A1;
+                    if (!FAST) dplus_coef(jv, nnx, q0, q1);
+                    const float pm3 = P[t][i0], pm2 = P[t][i1], pm1 = P[t][i2], pn = P[t][i3];
+                    const float dpx = fmaf(q0, pm1 - pm2, q1 * (pn - pm3));
+                    const float p_u1 = lane_up1(pm2);
+                    const float p_d1 = lane_dn1(pm2), p_d2 = lane_dn1(p_d1);
+                    const float dpz = fmaf(zp0, p_d1 - pm2, zp1 * (p_d2 - p_u1));
+                    float bx, bz, vx_old, vz_old;
+                    if (t == 0) {
+                        // BX, BZ at jv from the buoyancy (the host's fields, bit
+                        // for bit: cpu_stagger's averages and edge rules,
+                        // cpu_modelling_kernels.cpp:64-121, times dt/dx and dt/dz
+                        // in FP64, rounded without bias)
+                        const float b_1 = keep(cur.b);            // b(jv + 1)
+                        const bool last_row = r >= nnz - 1;
+                        const float bu1 = last_row ? 0.0f : 0.5f * (lane_dn1(b_0) + b_0);
+                        float bu2;
+                        if (FAST || jv < nnx - 1) {
+                            bu2 = 0.5f * (b_1 + b_0);
+                        } else {        // last column: bu1 of column nnx - 2
+                            const float b_m1 = ld(Bf, coff(jv - 1));
+                            bu2 = last_row ? 0.0f : 0.5f * (lane_dn1(b_m1) + b_m1);
+                        }
+                        bx = round_unbiased_prod(dtdx, bu2, cell_draw(RND_BX, jv, r));
+                        bz = round_unbiased_prod(dtdz, bu1, cell_draw(RND_BZ, jv, r));
+                        b_0 = b_1;
+                        vx_old = vxin[i0];
+                        vz_old = vzin[i0];
+                    } else {
+                        bx = BXw[t - 1][i0];
+                        bz = BZw[t - 1][i0];
+                        vx_old = VX[t - 1][i0];
+                        vz_old = VZ[t - 1][i0];
+                    }
+                    if (t < T - 1) {
+                        if (FAST) {
+                            VX[t][i3] = fmaf(bx, dpx, vx_old);
+                        } else {
+                            const int jc = clampc(jv);
+                            VX[t][i3] = fmaf(bx * prx[3 * nnx + jc], dpx, prx[2 * nnx + jc] * vx_old);
+                        }
+                        VZ[t][i3] = fmaf(bz * sbz, dpz, dbz * vz_old);
+                        BXw[t][i3] = bx;
+                        BZw[t][i3] = bz;
+                    } else if (rown && jv >= xa && jv < xb) {
+                        float vxn;
+                        if (FAST) {
+                            vxn = fmaf(bx, dpx, vx_old);
+                        } else {
+                            const int jc = clampc(jv);
+                            vxn = fmaf(bx * prx[3 * nnx + jc], dpx, prx[2 * nnx + jc] * vx_old);
+                        }
+                        const float vzn = fmaf(bz * sbz, dpz, dbz * vz_old);
+                        const unsigned id = (unsigned)jv * (unsigned)nnz * 4u + rc4;
+                        st4(so, 4 * id, PX[t][i1], PZ[t][i1], vxn, vzn);
+                    }
+                }
+            }
+            cur = load_col(min(c + PF, c_last));
+        };
+
+        using TT = std::integral_constant<bool, true>;
+        using FF = std::integral_constant<bool, false>;
+        using U0 = std::integral_constant<int, 0>;
+        using U1 = std::integral_constant<int, 1>;
+        using U2 = std::integral_constant<int, 2>;
+        using U3 = std::integral_constant<int, 3>;
+        int c0 = c_first;
+        for (; c0 + 3 <= c_last && c0 < fast_lo; c0 += 4) {
+            column(FF(), U0(), c0); column(FF(), U1(), c0);
+            column(FF(), U2(), c0); column(FF(), U3(), c0);
+        }
+        const int c_fast = min(fast_hi, c_last - 3);
+        for (; c0 <= c_fast; c0 += 4) {
+            column(TT(), U0(), c0); column(TT(), U1(), c0);
+            column(TT(), U2(), c0); column(TT(), U3(), c0);
+        }
+        for (; c0 <= c_last; c0 += 4) {
+            column(FF(), U0(), c0);
+            if (c0 + 1 > c_last) break;
+            column(FF(), U1(), c0);
+            if (c0 + 2 > c_last) break;
+            column(FF(), U2(), c0);
+            if (c0 + 3 > c_last) break;
+            column(FF(), U3(), c0);
+        }
+    }
+}
+
+struct StepNArgs {
+    const ShotDev *d;
+    int ns, vt, per, nt, step, in, out, bmask, bw;
+    int4 bs;
+    int ss, ng, xo, ring;
+};
+
+template <int T, int W, int PF, bool KZA, bool RING>
+static void launchn_t(dim3 g, hipStream_t s, const StepNArgs &a)
+{
+    hipLaunchKernelGGL((k_stepn<T, W, PF, KZA, RING>), g, dim3(64 * W), 0, s, a.d, a.ns, a.vt,
+                       a.per, a.nt, a.step, a.in, a.out, a.bmask, a.bw, a.bs, a.ss, a.ng, a.xo,
+                       a.ring);
+}
+
+template <int T, int W, bool RING>
+static void launchn_w(int pf, bool kza, dim3 g, hipStream_t s, const StepNArgs &a)
+{
+    if (pf == 2 && kza) launchn_t<T, W, 2, true, RING>(g, s, a);
+    else if (pf == 2) launchn_t<T, W, 2, false, RING>(g, s, a);
+    else if (kza) launchn_t<T, W, 4, true, RING>(g, s, a);
+    else launchn_t<T, W, 4, false, RING>(g, s, a);
+}
+
+// Launch of k_stepn<T> (RING: the recomputation form) with the variant's
+// waves per block and prefetch depth.  Each depth, and the ring form, is
+// instantiated in a translation unit of its own (sam2d_kernelsn_t*.hip) so
+// that the builds run in parallel.
+template <int T, bool RING>
+static hipError_t launchn_waves(int waves, int pf, bool kza, dim3 g, hipStream_t s,
+                                const StepNArgs &a)
+{
+    switch (waves) {
+    case 1: launchn_w<T, 1, RING>(pf, kza, g, s, a); break;
+    case 2: launchn_w<T, 2, RING>(pf, kza, g, s, a); break;
+    case 4: launchn_w<T, 4, RING>(pf, kza, g, s, a); break;
+    case 8: launchn_w<T, 8, RING>(pf, kza, g, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_stepn_t2(int waves, int pf, bool kza, dim3 g, hipStream_t s, const StepNArgs &a);
+hipError_t launch_stepn_t3(int waves, int pf, bool kza, dim3 g, hipStream_t s, const StepNArgs &a);
+hipError_t launch_stepn_t4(int waves, int pf, bool kza, dim3 g, hipStream_t s, const StepNArgs &a);
+hipError_t launch_stepn_t5(int waves, int pf, bool kza, dim3 g, hipStream_t s, const StepNArgs &a);
+hipError_t launch_stepn_t3_ring(int waves, int pf, bool kza, dim3 g, hipStream_t s,
+                                const StepNArgs &a);
+
+}  // namespace sam2d
