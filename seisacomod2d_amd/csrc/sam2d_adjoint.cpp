@@ -24,8 +24,8 @@
 namespace sam2d {
 
 struct AdjShotBuffers {
-    float *lam = nullptr;     // 2 adjoint slots x 4 fields, then the accumulator cells
-                              // (dJ/dkappa, dJ/dbu2, dJ/dbu1), all cell-interleaved
+    float *lam = nullptr;     // 2 adjoint slots of (L_px, L_vx, L_vz) cells, 2 of L_pz,
+                              // then the accumulator cells (dJ/dkappa, dJ/dbu2, dJ/dbu1)
     float *ckpt = nullptr;    // nckpt state slots (cell-interleaved)
     float *ring = nullptr;    // k triplet slots (ShotDev::ring)
     float *srcg = nullptr;
@@ -149,11 +149,12 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     const int ns = (int)plans.size();
     // Tile shape of the adjoint and of the single-step recomputation: the
     // "adjoint_variant" option, else the fastest shape measured for the C4
-    // gradient on MI355X (two steps: 4 waves x 32 columns, prefetch 2, best
-    // with 4 and with 32 shots resident; one step: 1 wave x 64 columns,
-    // prefetch 2; shapes of 16..64 columns are within 15 %).
+    // gradient on MI355X (two steps: 2 waves x 256 columns, prefetch 2, 1398
+    // us per pass against 1459-1599 for the prefetch-4 shapes, which need
+    // more than 256 registers, and 1465 for 4 waves x 32 columns; one step:
+    // 1 wave x 64 columns, prefetch 2).
     const int variant = ctx->adjoint_variant >= 0 ? ctx->adjoint_variant
-                        : ctx->adjoint_steps == 2 ? variant_index(4, 32, 2)
+                        : ctx->adjoint_steps == 2 ? variant_index(2, 256, 2)
                                                   : variant_index(1, 64, 2);
     int ntiles = 0;
     SAM2D_TRY(stage_batch(ctx, plans, variant, &ntiles, 2));
@@ -167,8 +168,8 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         ShotPlan &p = plans[s];
         const DomainHost &h = p.dom->host;
         const size_t N = (size_t)h.nnx * h.nnz;
-        // the adjoint kernels address the cell-interleaved adjoint state with
-        // 32-bit byte offsets (16 bytes per cell)
+        // the adjoint and recomputation kernels address cells with 32-bit
+        // byte offsets (up to 16 bytes per cell)
         if (N >= ((size_t)1 << 28))
             return fail(SAM2D_ERR_PARAM, "grid of %zu cells too large for the gradient", N);
         if (!A.bufs[s]) A.bufs[s].reset(new AdjShotBuffers());
@@ -189,6 +190,8 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         SAM2D_TRY(grow((void **)&b.rptr, b.cap[4], (ng1 + 1) * sizeof(int)));
         SAM2D_TRY(grow((void **)&b.rid, b.cap[5], ng1 * sizeof(int)));
         SAM2D_TRY(grow((void **)&b.res, b.cap[6], ng1 * ta.real_nt * sizeof(float)));
+        // Two adjoint slots of (L_px, L_vx, L_vz) cells, two of L_pz, then the
+        // accumulator cells.
         float *lam = b.lam, *acc = b.lam + 8 * N;
         HIP_TRY(launch_fill(lam, 11 * N, 0.0f, ctx->stream));
 
@@ -258,8 +261,9 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         d.srcg = b.srcg;
         d.coef = p.dom->ADJ;
         d.prof_x = fd.prof_x; d.prof_z = fd.prof_z;
-        d.ring = b.ring; d.lam = lam; d.ncell = N;
+        d.ring = b.ring; d.lam = lam; d.lpz = lam + 6 * N; d.ncell = N;
         d.acc = acc;
+        ones_range(h.prof_z, h.nnz, d.zin_lo, d.zin_hi);
         d.xin_lo = fd.xin_lo; d.xin_hi = fd.xin_hi;
         d.ncells = ncells[s];
         d.rcell = b.rcell; d.rptr = b.rptr; d.rid = b.rid; d.res = b.res;

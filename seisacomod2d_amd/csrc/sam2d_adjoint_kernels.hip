@@ -49,6 +49,15 @@ namespace sam2d {
 // adjoints, which form the same terms in different places, give the same bits.
 #pragma clang fp contract(off)
 
+// The sponge frame of a domain: cells where an x- or z-profile differs from
+// 1.  Outside it the p-sponge factors are exactly 1, so L_px and L_pz take the
+// same values (they start at zero and receive the same increments), and only
+// L_px is stored.
+__device__ __forceinline__ bool in_frame(const AdjDev &d, int col, int row)
+{
+    return col < d.xin_lo || col > d.xin_hi || row < d.zin_lo || row > d.zin_hi;
+}
+
 // Residual injection before backward step n, for shots without a residual
 // band: for each distinct receiver cell,
 // the residuals of its receivers (sample rec_k, receiver order) are summed and
@@ -61,15 +70,17 @@ __global__ void k_inject(const AdjDev *__restrict__ shots, int nshots, int lam, 
     for (int s = 0; s < nshots; ++s) {
         const AdjDev &d = shots[s];
         if (d.rband) continue;                        // injected inside k_adjoint
-        // cell-interleaved adjoint state: L_px, L_pz of cell c at 4c, 4c+1
-        SAM2D_GLOBAL float *L = gp(d.lam + 4 * (size_t)lam * d.ncell);
+        // L_px of cell c at 3c; L_pz in the sponge frame only
+        SAM2D_GLOBAL float *L = gp(d.lam + 3 * (size_t)lam * d.ncell);
+        SAM2D_GLOBAL float *Z = gp(d.lpz + (size_t)lam * d.ncell);
         for (int u = gtid; u < d.ncells; u += gstride) {
             float sum = 0.0f;
             for (int e = d.rptr[u]; e < d.rptr[u + 1]; e++)
                 sum += gp(d.res)[(size_t)d.rid[e] * d.real_nt + rec_k];
-            const size_t c = 4 * (size_t)d.rcell[u];
-            L[c] += sum;
-            L[c + 1] += sum;
+            const int c = d.rcell[u];
+            const int col = c / d.nnz, row = c - col * d.nnz;
+            L[3 * (size_t)c] += sum;
+            if (in_frame(d, col, row)) Z[c] += sum;
         }
     }
 }
@@ -131,10 +142,11 @@ hipError_t launch_pack_coef(const float *kx, const float *kz, const float *bx, c
     return hipGetLastError();
 }
 
-// The adjoint state is cell-interleaved (L_px, L_pz, L_vx, L_vz of cell i at
-// 4i .. 4i+3): a column's state arrives as one 16-byte load per lane at
-// column c+1 (kept whole in the prefetch slot) and leaves as one 16-byte
-// store at the phase-B column, L_px and L_pz kept from phase A.  The forward
+// The adjoint state is cell-interleaved (L_px, L_vx, L_vz of cell i at
+// 3i .. 3i+2, L_pz apart and only in the sponge frame): a column's state
+// arrives as one 12-byte load per lane at column c+1 (kept whole in the
+// prefetch slot) and leaves as one 12-byte store at the phase-B column, L_px
+// kept from phase A; frame lanes add a 4-byte load and store.  The forward
 // triplet of step n (p[n], vx[n-1], vz[n-1]) and the coefficient cell (KX,
 // BX, BZ[, KZ]) arrive at column c+1 as well, their column-c parts waiting a
 // column in windows; the accumulator cell (dJ/dkappa, dJ/dbu2, dJ/dbu1)
@@ -145,7 +157,8 @@ using CoefCell = typename std::conditional<KZA, ftri, fquad>::type;
 
 template <bool KZA>
 struct AdjCol {
-    fquad lam;                  // column c+1: adjoint state
+    ftri lam;                   // column c+1: adjoint L_px, L_vx, L_vz
+    float lpz;                  //             L_pz (sponge frame)
     ftri fw;                    //             p[n], vx[n-1], vz[n-1]
     CoefCell<KZA> cf;           //             KX, BX, BZ[, KZ]
     ftri acc;                   // column c:   dJ/dkappa, dJ/dbu2, dJ/dbu1
@@ -161,7 +174,6 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
     static_assert(PF == 2 || PF == 4, "PF must divide the unroll of 4");
     using gcchar = const SAM2D_GLOBAL char;
     using gchar = SAM2D_GLOBAL char;
-    constexpr unsigned CW = KZA ? 3u : 4u;            // floats per coefficient cell
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
@@ -190,8 +202,10 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         // Adjoint state, forward ring, coefficients and accumulators never
         // overlap.  The accumulators are read at a cell and written back there
         // by the same lane only.
-        gcchar *__restrict__ Lin = (gcchar *)(d.lam + 4 * (size_t)lam_in * N);
-        gchar *__restrict__ Lout = (gchar *)(d.lam + 4 * (size_t)lam_out * N);
+        gcchar *__restrict__ Lin = (gcchar *)(d.lam + 3 * (size_t)lam_in * N);
+        gchar *__restrict__ Lout = (gchar *)(d.lam + 3 * (size_t)lam_out * N);
+        gcchar *__restrict__ Zin = (gcchar *)(d.lpz + (size_t)lam_in * N);
+        gchar *__restrict__ Zout = (gchar *)(d.lpz + (size_t)lam_out * N);
         gcchar *__restrict__ FW = (gcchar *)(d.ring + 3 * (size_t)fslot * N);
         gcchar *__restrict__ CF = (gcchar *)d.coef;
         gchar *__restrict__ ACC = (gchar *)d.acc;
@@ -200,11 +214,6 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
 
         auto ld4 = [](gcchar *__restrict__ base, unsigned off) -> fquad {
             return *(const SAM2D_GLOBAL fquad *__restrict__)(base + off);
-        };
-        auto st4 = [](gchar *__restrict__ base, unsigned off, float a, float b, float c, float e) {
-            fquad v;
-            v.x = a; v.y = b; v.z = c; v.w = e;
-            *(SAM2D_GLOBAL fquad *__restrict__)(base + off) = v;
         };
         auto ldcf = [&](unsigned off) -> CoefCell<KZA> {
             if constexpr (KZA) return ld3(CF, 3 * off);
@@ -216,6 +225,18 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             return (unsigned)min(max(col, 0), nnx - 1) * (unsigned)nnz * 4u + rc4;
         };
         auto clampc = [&](int col) -> int { return min(max(col, 0), nnx - 1); };
+        // Sponge-frame lanes of a column (clamped as the loads are).
+        const bool row_frame = r < d.zin_lo || r > d.zin_hi;
+        auto frame = [&](int col) -> bool {
+            const int cc = clampc(col);
+            return row_frame || cc < d.xin_lo || cc > d.xin_hi;
+        };
+        auto ldf = [](gcchar *__restrict__ base, unsigned off) -> float {
+            return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
+        };
+        auto stz = [](gchar *__restrict__ base, unsigned off, float val) {
+            *(SAM2D_GLOBAL float *__restrict__)(base + off) = val;
+        };
 
         const float daz = prz[rc], skz = prz[nnz + rc];
         const float dbz = prz[2 * nnz + rc], sbz = prz[3 * nnz + rc];
@@ -243,7 +264,9 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         auto load_col = [&](int c) -> AdjCol<KZA> {
             AdjCol<KZA> o;
             const unsigned o1 = coff(c + 1), o0 = coff(c);
-            o.lam = ld4(Lin, 4 * o1);
+            o.lam = ld3(Lin, 3 * o1);
+            o.lpz = 0.0f;
+            if (frame(c + 1)) o.lpz = ldf(Zin, o1);
             o.fw = ld3(FW, 3 * o1);
             o.cf = ldcf(o1);
             o.acc = ld3(ACC, 3 * o0);
@@ -281,8 +304,9 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         for (int k = 0; k < 3; k++) {
             const int col = xa - 3 + k;
             const unsigned o = coff(col);
-            const fquad lv = ld4(Lin, 4 * o);
-            LPX[k] = lv.x; LPZ[k] = lv.y; LVX[k] = lv.z; LVZ[k] = lv.w;
+            const ftri lv = ld3(Lin, 3 * o);
+            LPX[k] = lv.x; LVX[k] = lv.y; LVZ[k] = lv.z;
+            LPZ[k] = frame(col) ? ldf(Zin, o) : lv.x;
             const ftri fw = ld3(FW, 3 * o);
             FP[k] = fw.x; FVX[k] = fw.y; FVZ[k] = fw.z;
             const CoefCell<KZA> cf = ldcf(o);
@@ -304,9 +328,9 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
 
             // ---- phase A at column c
             LPX[i3] = keep(cur.lam.x);
-            LPZ[i3] = keep(cur.lam.y);
-            LVZ[i3] = keep(cur.lam.w);
-            const float lvx_n = keep(cur.lam.z);
+            LPZ[i3] = frame(c + 1) ? keep(cur.lpz) : LPX[i3];
+            LVZ[i3] = keep(cur.lam.z);
+            const float lvx_n = keep(cur.lam.y);
             LVX[i3] = lvx_n;
             FP[i3] = keep(cur.fw.x);
             FVX[i3] = keep(cur.fw.y);
@@ -384,7 +408,8 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
                 const float dpz = fmaf(zp0, q_d1 - pj, zp1 * (q_d2 - q_u1));
                 const unsigned id = (unsigned)j * (unsigned)nnz * 4u + rc4;
                 const float lvz_j = LVZ[i0];
-                st4(Lout, 4 * id, OPX[i1], OPZ[i1], fmaf(dbx, lvx_j, tx), fmaf(dbz, lvz_j, tz));
+                st3(Lout, 3 * id, OPX[i1], fmaf(dbx, lvx_j, tx), fmaf(dbz, lvz_j, tz));
+                if (frame(j)) stz(Zout, id, OPZ[i1]);
                 st3(ACC, 3 * id, GK[i0], GBX[i0] + lvx_j * dpx * (dtdx * sbx),
                     GBZ[i0] + lvz_j * dpz * (dtdz * sbz));
             }
@@ -435,7 +460,8 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
 // loaded at column c-2.
 template <bool KZA>
 struct AdjCol2 {
-    fquad lam;                  // column c+1: adjoint state (cell-interleaved)
+    ftri lam;                   // column c+1: adjoint L_px, L_vx, L_vz
+    float lpz;                  //             L_pz (sponge frame)
     ftri fw1;                   //             forward triplet of step n
     CoefCell<KZA> cf;           //             KX, BX, BZ[, KZ]
     ftri acc;                   // column c:   dJ/dkappa, dJ/dbu2, dJ/dbu1
@@ -478,8 +504,10 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         const unsigned rc4 = rc * 4u;
         const size_t N = d.ncell;
 
-        gcchar *__restrict__ Lin = (gcchar *)(d.lam + 4 * (size_t)lam_in * N);
-        gchar *__restrict__ Lout = (gchar *)(d.lam + 4 * (size_t)lam_out * N);
+        gcchar *__restrict__ Lin = (gcchar *)(d.lam + 3 * (size_t)lam_in * N);
+        gchar *__restrict__ Lout = (gchar *)(d.lam + 3 * (size_t)lam_out * N);
+        gcchar *__restrict__ Zin = (gcchar *)(d.lpz + (size_t)lam_in * N);
+        gchar *__restrict__ Zout = (gchar *)(d.lpz + (size_t)lam_out * N);
         gcchar *__restrict__ FW1 = (gcchar *)(d.ring + 3 * (size_t)fslot * N);
         gcchar *__restrict__ FW2 = (gcchar *)(d.ring + 3 * (size_t)(fslot - 1) * N);
         gcchar *__restrict__ CF = (gcchar *)d.coef;
@@ -487,14 +515,9 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);
         const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
 
-        // Adjoint state: byte offset 4x that of one field.
+        // Coefficient cells of four floats: byte offset 4x that of one field.
         auto ld4 = [](gcchar *__restrict__ base, unsigned off) -> fquad {
             return *(const SAM2D_GLOBAL fquad *__restrict__)(base + off);
-        };
-        auto st4 = [](gchar *__restrict__ base, unsigned off, float a, float b, float c, float e) {
-            fquad v;
-            v.x = a; v.y = b; v.z = c; v.w = e;
-            *(SAM2D_GLOBAL fquad *__restrict__)(base + off) = v;
         };
         auto ldcf = [&](unsigned off) -> CoefCell<KZA> {
             if constexpr (KZA) return ld3(CF, 3 * off);
@@ -504,6 +527,18 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
             return (unsigned)min(max(col, 0), nnx - 1) * (unsigned)nnz * 4u + rc4;
         };
         auto clampc = [&](int col) -> int { return min(max(col, 0), nnx - 1); };
+        // Sponge-frame lanes of a column (clamped as the loads are).
+        const bool row_frame = r < d.zin_lo || r > d.zin_hi;
+        auto frame = [&](int col) -> bool {
+            const int cc = clampc(col);
+            return row_frame || cc < d.xin_lo || cc > d.xin_hi;
+        };
+        auto ldf = [](gcchar *__restrict__ base, unsigned off) -> float {
+            return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
+        };
+        auto stz = [](gchar *__restrict__ base, unsigned off, float val) {
+            *(SAM2D_GLOBAL float *__restrict__)(base + off) = val;
+        };
         auto p0_at = [&](int i) { float a, b; dplus_coef(i, nnx, a, b); return a; };
         auto p1_at = [&](int i) { float a, b; dplus_coef(i, nnx, a, b); return b; };
         auto m0_at = [&](int i) { float a, b; dminus_coef(i, nnx, a, b); return a; };
@@ -539,7 +574,9 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         auto load_col = [&](int c) -> AdjCol2<KZA> {
             AdjCol2<KZA> o;
             const unsigned o1 = coff(c + 1), o0 = coff(c), oj = coff(c - 2);
-            o.lam = ld4(Lin, 4 * o1);
+            o.lam = ld3(Lin, 3 * o1);
+            o.lpz = 0.0f;
+            if (frame(c + 1)) o.lpz = ldf(Zin, o1);
             o.fw1 = ld3(FW1, 3 * o1);
             o.cf = ldcf(o1);
             o.acc = ld3(ACC, 3 * o0);
@@ -585,8 +622,9 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         for (int k = 0; k < 3; k++) {
             const int col = c_first - 2 + k;
             const unsigned o = coff(col);
-            const fquad lv = ld4(Lin, 4 * o);
-            LPXn[k] = lv.x; LPZn[k] = lv.y; LVX0[k] = lv.z; LVZn[k] = lv.w;
+            const ftri lv = ld3(Lin, 3 * o);
+            LPXn[k] = lv.x; LVX0[k] = lv.y; LVZn[k] = lv.z;
+            LPZn[k] = frame(col) ? ldf(Zin, o) : lv.x;
             const ftri fw = ld3(FW1, 3 * o);
             FP1n[k] = fw.x; FVX1[k] = fw.y; FVZ1n[k] = fw.z;
             const CoefCell<KZA> cf = ldcf(o);
@@ -609,9 +647,9 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
             // ---- A1: step n, phase A at column c
             {
                 LPXn[i3] = keep(cur.lam.x);
-                LPZn[i3] = keep(cur.lam.y);
-                LVZn[i3] = keep(cur.lam.w);
-                const float lvx_n = keep(cur.lam.z);
+                LPZn[i3] = frame(c + 1) ? keep(cur.lpz) : LPXn[i3];
+                LVZn[i3] = keep(cur.lam.z);
+                const float lvx_n = keep(cur.lam.y);
                 LVX0[i3] = lvx_n;
                 FP1n[i3] = keep(cur.fw1.x);
                 FVX1[i3] = keep(cur.fw1.y);
@@ -775,7 +813,8 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                     const float dpx = fmaf(xp0, PN2[i2] - pj, xp1 * (PN2[i3] - PN2[i0]));
                     const float dpz = fmaf(zp0, q_d1 - pj, zp1 * (q_d2 - q_u1));
                     const unsigned id = (unsigned)j2 * (unsigned)nnz * 4u + rc4;
-                    st4(Lout, 4 * id, OPA[i1], OPZA[i1], fmaf(dbx, lvx_j, tx), fmaf(dbz, lvz_j, tz));
+                    st3(Lout, 3 * id, OPA[i1], fmaf(dbx, lvx_j, tx), fmaf(dbz, lvz_j, tz));
+                    if (frame(j2)) stz(Zout, id, OPZA[i1]);
                     st3(ACC, 3 * id, GK2[i1], GBXW[i0] + lvx_j * dpx * (dtdx * sbx),
                         GBZW[i0] + lvz_j * dpz * (dtdz * sbz));
                 }

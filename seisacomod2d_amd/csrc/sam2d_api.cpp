@@ -201,9 +201,9 @@ int pick_variant(sam2d_ctx *ctx)
     return ctx->default_variant;
 }
 
-// Columns [lo, hi] around the middle where all four x-profiles
-// (prof_x[4][nnx]) are exactly 1; lo > hi when the middle column is not.
-static void ones_range(const std::vector<float> &prof_x, int nnx, int &lo, int &hi)
+// Indices [lo, hi] around the middle where all four profiles of a [4][nnx]
+// table are exactly 1; lo > hi when the middle index is not.
+void ones_range(const std::vector<float> &prof_x, int nnx, int &lo, int &hi)
 {
     auto ones = [&](int c) {
         for (int k = 0; k < 4; k++)

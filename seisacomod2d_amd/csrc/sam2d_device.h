@@ -73,8 +73,12 @@ struct AdjDev {
                            //   [3i .. 3i+2] ([4i .. 4i+3] with KZ last)
     const float *prof_x, *prof_z;
     const float *ring;     // forward ring of triplets (ShotDev::ring)
-    float *lam;            // adjoint state: 2 slots of 4*ncell floats, cell-interleaved
-                           // (L_px, L_pz, L_vx, L_vz of cell i at 4i .. 4i+3)
+    float *lam;            // adjoint state: 2 slots of 3*ncell floats, L_px, L_vx, L_vz
+                           //   of cell i at [3i .. 3i+2]
+    float *lpz;            // L_pz: 2 slots of ncell floats, read and written only in the
+                           //   sponge frame; elsewhere L_pz equals L_px (both sponge
+                           //   factors are 1 there and both take the same increments)
+    int zin_lo, zin_hi;    // rows where all four z-profiles are exactly 1
     size_t ncell;
     float *acc;            // dJ/dkappa, dJ/dbu2 (x-staggered buoyancy), dJ/dbu1
                            //   (z-staggered) of cell i at [3i .. 3i+2]

@@ -147,6 +147,9 @@ int enqueue_steps(sam2d_ctx *ctx, int nshots, int ntiles, int variant, int n0, i
 int enqueue_steps_on(sam2d_ctx *ctx, const ShotDev *desc, int ngroups, int nshots, int ntiles,
                      int variant, int n0, int n1, int max_ng, int start_slot, int *end_slot);
 void tile_shape(int nnz, int nnx, int waves, int chunk, int own_rows, int &ngroups, int &nchunks);
+// Indices [lo, hi] around the middle of an n-point axis where all four
+// sponge profiles of prof[4][n] are exactly 1 (lo > hi: none).
+void ones_range(const std::vector<float> &prof, int n, int &lo, int &hi);
 // Descriptor of one shot for `variant` with march length `chunk`, tiles from
 // tile_begin; the device buffers are left for the caller to set.
 void describe_shot(const ShotPlan &p, int variant, int chunk, int tile_begin, int nts, int real_nt,
