@@ -16,7 +16,9 @@ Default run, one JSON line:
     batched launch against the per-shot loop on the same windows.
   * "gradient": checkpointed adjoint gradient on the salt model, 32 windowed
     shots, 8000 steps, checkpoints every 100 steps: seconds per shot and the
-    ratio to the batched forward and to the per-shot forward of the same shots.
+    ratio to the batched forward and to the per-shot forward of the same shots;
+    also the same gradient in the two halves an FWI iteration runs (modelling
+    that keeps the checkpoints, then the backward pass from them).
   * "cpu_baseline": the compiled reference (src_cpu) on this host.
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-extra]
@@ -365,7 +367,25 @@ def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False, a
             t0 = time.perf_counter()
             gvp, grho = m.gradient(sh, adj, ckpt_k=k, windows=wins)
             t_g = time.perf_counter() - t0
-            res.append({"workload": label, "ckpt_k": k, "shots": len(sh), "adjoint_steps": adj_steps,
+            # The same gradient in its two halves, as an FWI iteration runs it:
+            # modelling that keeps the checkpoints, then the backward pass.
+            fwi = {}
+            try:
+                t0 = time.perf_counter()
+                m.forward_checkpointed(sh, ckpt_k=k, windows=wins)
+                t_fc = time.perf_counter() - t0
+                t0 = time.perf_counter()
+                m.gradient_checkpointed(adj)
+                t_b = time.perf_counter() - t0
+                fwi = {"forward_checkpointed_s_per_shot": round(t_fc / len(sh), 4),
+                       "backward_s_per_shot": round(t_b / len(sh), 4),
+                       "fwi_iteration_s_per_shot": round((t_fc + t_b) / len(sh), 4),
+                       "backward_ratio_to_batched_forward": round(t_b / t_fwd, 3),
+                       "backward_ratio_to_per_shot_forward": round(t_b / t_plain, 3)}
+            except sam.SAM2DError as e:            # the shots exceed one resident batch
+                fwi = {"forward_checkpointed": str(e)}
+            res.append(dict({"workload": label, "ckpt_k": k, "shots": len(sh),
+                        "adjoint_steps": adj_steps,
                         "forward_batched_s_per_shot": round(t_fwd / len(sh), 4),
                         "forward_per_shot_loop_s_per_shot": round(t_plain / len(sh), 4),
                         "gradient_s_per_shot": round(t_g / len(sh), 4),
@@ -373,9 +393,11 @@ def _grad_case(sam, grid, vtrue, v0, rho, sh, wins, ks, label, no_graph=False, a
                         "ratio_to_batched_forward": round(t_g / t_fwd, 3),
                         "ratio_to_per_shot_forward": round(t_g / t_plain, 3),
                         "misfit": float(sum(0.5 * np.sum(a.astype(np.float64) ** 2) for a in adj)),
-                        "grad_norm_vp": float(np.linalg.norm(gvp))})
+                        "grad_norm_vp": float(np.linalg.norm(gvp))}, **fwi))
             print(f"# {label} k={k}: gradient {t_g:.2f} s, batched forward {t_fwd:.2f} s, "
-                  f"per-shot forward {t_plain:.2f} s", file=sys.stderr, flush=True)
+                  f"per-shot forward {t_plain:.2f} s, backward from checkpoints "
+                  f"{fwi.get('backward_s_per_shot', float('nan')) * len(sh):.2f} s",
+                  file=sys.stderr, flush=True)
     return res
 
 

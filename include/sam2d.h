@@ -170,6 +170,25 @@ int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
                    const int *windows, const float *adj, int ckpt_k,
                    float *seis, float *g_vp, float *g_rho);
 
+/* sam2d_gradient in two halves, for FWI loops that model the data before
+ * they know the residual (no reference counterpart).  The first is forward
+ * modelling: the seismograms of the shots (as sam2d_forward_windows gives
+ * them, the same bits; windows may be NULL) from the gradient's forward
+ * sweep, which also keeps the state entering every ckpt_k-th step on the
+ * device.  Every shot stays resident, so the call returns SAM2D_ERR_MEMORY
+ * when the shots' checkpoints, ring and adjoint state exceed the gradient
+ * budget ("hbm_budget_gb"); sam2d_gradient runs such sets in batches. */
+int sam2d_forward_checkpointed(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
+                               const int *windows, int ckpt_k, float *seis);
+/* The second half: the backward pass of sam2d_gradient from the checkpoints
+ * of this context's last sam2d_forward_checkpointed call, for adjoint sources
+ * adj of those shots (layout of its seis); g_vp, g_rho as sam2d_gradient, the
+ * same bits.  Other modelling calls in between leave the checkpoints in
+ * place, and the pass may be repeated with other sources; a sam2d_gradient
+ * call drops them (SAM2D_ERR_PARAM then). */
+int sam2d_gradient_checkpointed(sam2d_ctx *ctx, const float *adj, float *g_vp,
+                                float *g_rho);
+
 /* Tuning and options.  Keys: "graph" (0/1, default 1), "variant" (-1 = auto,
  * else a kernel variant index), "reset_quirk" (1 = each shot of the per-shot
  * host-output loop, sam2d_forward, starts from the final state of the shot

@@ -79,6 +79,9 @@ def lib():
         L.sam2d_forward_windows.argtypes = [vp, P(_Shot), ctypes.c_int, i32p, f32p]
         L.sam2d_gradient.argtypes = [vp, P(_Shot), ctypes.c_int, i32p, f32p, ctypes.c_int,
                                      f32p, f32p, f32p]
+        L.sam2d_forward_checkpointed.argtypes = [vp, P(_Shot), ctypes.c_int, i32p, ctypes.c_int,
+                                                 f32p]
+        L.sam2d_gradient_checkpointed.argtypes = [vp, f32p, f32p, f32p]
         L.sam2d_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
         L.sam2d_get_option.argtypes = [vp, ctypes.c_char_p, P(ctypes.c_int)]
         L.sam2d_autotune.argtypes = [vp, i32p, ctypes.c_int, ctypes.c_int, i32p]
@@ -272,10 +275,35 @@ class Modeller:
         seis = self._out(shots) if want_seis else None
         w, wp = _windows_c(windows, len(shots))
         fp = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if x is not None else None
+        # the library drops the checkpoints of forward_checkpointed
+        self.checkpoint_id = getattr(self, "checkpoint_id", 0) + 1
         _check(lib().sam2d_gradient(self._h, _shots_c(shots), len(shots), wp, fp(a), int(ckpt_k),
                                     fp(seis), fp(gvp), fp(grho)))
         if want_seis:
             return gvp, grho, self._split(seis, shots)
+        return gvp, grho
+
+    def forward_checkpointed(self, shots, ckpt_k=100, windows=None) -> List[np.ndarray]:
+        """Seismograms of the shots (the bits of forward_windows) from the
+        gradient's forward sweep; its checkpoints stay on the device for
+        gradient_checkpointed."""
+        seis = self._out(shots)
+        w, wp = _windows_c(windows, len(shots))
+        self.checkpoint_id = getattr(self, "checkpoint_id", 0) + 1
+        _check(lib().sam2d_forward_checkpointed(self._h, _shots_c(shots), len(shots), wp,
+                                                int(ckpt_k),
+                                                seis.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return self._split(seis, shots)
+
+    def gradient_checkpointed(self, adj: List[np.ndarray]):
+        """dL/dvp, dL/drho from the checkpoints of the last forward_checkpointed
+        call, for adjoint sources adj of its shots (the bits of gradient).
+        checkpoint_id names the call whose checkpoints are held."""
+        a = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float32).ravel() for x in adj]))
+        gvp = np.zeros((self.grid.nx, self.grid.nz), np.float32)
+        grho = np.zeros_like(gvp)
+        fp = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        _check(lib().sam2d_gradient_checkpointed(self._h, fp(a), fp(gvp), fp(grho)))
         return gvp, grho
 
     def tuning(self):

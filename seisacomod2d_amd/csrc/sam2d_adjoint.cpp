@@ -52,6 +52,12 @@ struct AdjointState {
     int cap_fwd = 0;
     ShotDev *d_ring = nullptr;     // recomputation descriptors (three-step tile shapes)
     int cap_ring = 0;
+    // Checkpoints of the last sam2d_forward_checkpointed call, held in bufs
+    // for sam2d_gradient_checkpointed (a sam2d_gradient call reuses bufs and
+    // drops them).
+    bool held = false;
+    std::vector<ShotPlan> held_plans;
+    int held_k = 0;
     ~AdjointState() { dev_free(d_desc); dev_free(d_desc2); dev_free(d_fwd); dev_free(d_ring); }
 };
 
@@ -140,10 +146,19 @@ static int ring_form(int v)
     return v;
 }
 
+// Phases of a gradient batch: the forward sweep with checkpoints, and the
+// backward pass (recomputation, adjoint, chain rule) from them.
+enum { PH_SWEEP = 1, PH_BACKWARD = 2 };
+
+// One batch: the phases asked for.  adj (backward): the adjoint sources per
+// shot; seis_out (sweep, optional): the seismograms per shot.  A backward
+// pass without the sweep reads the checkpoints an earlier sweep of the same
+// plans and k left in the batch's buffers.
 static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const float *const *adj,
                           int k, float *const *seis_out, std::vector<double> &g_vp,
-                          std::vector<double> &g_rho)
+                          std::vector<double> &g_rho, int phases = PH_SWEEP | PH_BACKWARD)
 {
+    const bool backward = phases & PH_BACKWARD;
     AdjointState &A = *ctx->adj;
     const TimeAxis &ta = ctx->ta;
     const int ns = (int)plans.size();
@@ -193,7 +208,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         // Two adjoint slots of (L_px, L_vx, L_vz) cells, two of L_pz, then the
         // accumulator cells.
         float *lam = b.lam, *acc = b.lam + 8 * N;
-        HIP_TRY(launch_fill(lam, 11 * N, 0.0f, ctx->stream));
+        if (backward) HIP_TRY(launch_fill(lam, 11 * N, 0.0f, ctx->stream));
 
         // Source-weight per step and receivers by column (CSR).
         std::vector<float> srcg(std::max(1, ta.nts), 0.0f);
@@ -222,7 +237,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         SAM2D_TRY(upload(b.rcell, rcell.data(), rcell.size() * 4, ctx->stream));
         SAM2D_TRY(upload(b.rptr, rptr.data(), rptr.size() * 4, ctx->stream));
         SAM2D_TRY(upload(b.rid, rid.data(), (size_t)p.ng * 4, ctx->stream));
-        SAM2D_TRY(upload(b.res, adj[s], (size_t)p.ng * ta.real_nt * 4, ctx->stream));
+        if (backward) SAM2D_TRY(upload(b.res, adj[s], (size_t)p.ng * ta.real_nt * 4, ctx->stream));
         // Dense residual band over the receiver rows, [real_nt][band_h][nnx]:
         // each receiver cell holds the sum of its receivers' residuals (in the
         // order above), every other cell -0, so adding it is exact elsewhere.
@@ -235,6 +250,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         const int band_h = p.ng > 0 ? rhi - rlo + 1 : 0;
         const size_t band_n = (size_t)ta.real_nt * band_h * h.nnx;
         bool use_band = p.ng > 0 && band_n * sizeof(float) <= ((size_t)1 << 30);
+        if (!backward) use_band = false;      // no residuals yet
         if (use_band) {
             // Built on the device from the residuals: a map from band cell to
             // receiver cell (or -1), then one pass over the band.
@@ -397,7 +413,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     //    (cell-interleaved state, as the forward-modelling calls)
     const bool want_seis = seis_out != nullptr;
     int cur = 0;
-    for (int m = 0; m < nseg; m++) {
+    for (int m = 0; m < nseg && (phases & PH_SWEEP); m++) {
         SAM2D_TRY(save_or_restore(m, cur, true));
         const int n0 = m * k, n1 = std::min(ta.fd_nt, n0 + k);
         SAM2D_TRY(enqueue_steps_on(ctx, fwd_desc, fwd_groups, ns, fwd_tiles, fwd_variant, n0, n1,
@@ -405,7 +421,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     }
     // 2. backward by segments
     int lam = 0;
-    for (int m = nseg - 1; m >= 0; m--) {
+    for (int m = nseg - 1; m >= 0 && backward; m--) {
         const int n0 = m * k, n1 = std::min(ta.fd_nt, n0 + k);
         SAM2D_TRY(save_or_restore(m, 0, false));
         // Recompute the segment, step n's triplets into ring slot n - n0:
@@ -444,8 +460,9 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.kernel_ms += ms;
-    ctx->stats.launches += (long long)ta.fd_nt * 2 +
-                           (ta.fd_nt + variant_steps(fwd_variant) - 1) / variant_steps(fwd_variant);
+    if (backward) ctx->stats.launches += (long long)ta.fd_nt * 2;
+    if (phases & PH_SWEEP)
+        ctx->stats.launches += (ta.fd_nt + variant_steps(fwd_variant) - 1) / variant_steps(fwd_variant);
     ctx->stats.variant = variant;
     ctx->stats.steps_per_launch = 1;
 
@@ -454,6 +471,10 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
         const DomainHost &h = plans[s].dom->host;
         const size_t N = (size_t)h.nnx * h.nnz;
         ctx->stats.cell_updates += (double)N * ta.fd_nt;
+        if (want_seis && plans[s].ng)
+            HIP_TRY(hipMemcpy(seis_out[s], plans[s].seis_dev,
+                              (size_t)plans[s].ng * ta.real_nt * sizeof(float), hipMemcpyDeviceToHost));
+        if (!backward) continue;
         std::vector<float> g(3 * N);
         HIP_TRY(hipMemcpy(g.data(), A.bufs[s]->lam + 8 * N, 3 * N * sizeof(float),
                           hipMemcpyDeviceToHost));
@@ -464,9 +485,6 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
             gbz[i] = g[3 * i + 2];
         }
         chain_to_model(ctx, h, gk, gbx, gbz, g_vp, g_rho);
-        if (want_seis && plans[s].ng)
-            HIP_TRY(hipMemcpy(seis_out[s], plans[s].seis_dev,
-                              (size_t)plans[s].ng * ta.real_nt * sizeof(float), hipMemcpyDeviceToHost));
     }
     return SAM2D_OK;
 }
@@ -479,6 +497,44 @@ void sam2d_adjoint_release(sam2d_ctx *ctx)
 {
     delete ctx->adj;
     ctx->adj = nullptr;
+}
+
+// Device bytes a gradient batch may use: the "hbm_budget_gb" option, else
+// the device memory that is free or held by this context's reusable buffers,
+// less a reserve of 5 % of the device (at least 4 GiB), and at most 90 % of
+// the device.
+static int gradient_budget(sam2d_ctx *ctx, size_t *out)
+{
+    if (ctx->hbm_budget_gb > 0) {
+        *out = (size_t)ctx->hbm_budget_gb << 30;
+        return SAM2D_OK;
+    }
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    size_t held = 0;
+    for (auto &b : ctx->bufs)
+        if (b) held += b->cap_state * sizeof(float);
+    for (auto &b : ctx->adj->bufs)
+        if (b)
+            for (size_t c : b->cap) held += c;
+    const size_t reserve = std::max<size_t>((size_t)4 << 30, total_b / 20);
+    const size_t avail = free_b + held > reserve ? free_b + held - reserve : 0;
+    *out = std::min(avail, total_b / 10 * 9);
+    return SAM2D_OK;
+}
+
+// Per-shot pointers into a concatenated [sum nrec][real_nt] array.
+template <class T>
+static std::vector<T *> per_shot(T *base, const sam2d_shot *shots, int nshots, int real_nt)
+{
+    std::vector<T *> out(nshots, nullptr);
+    size_t off = 0;
+    for (int s = 0; s < nshots; s++) {
+        out[s] = base ? base + off : nullptr;
+        off += (size_t)shots[s].nrec * real_nt;
+    }
+    return out;
 }
 
 extern "C" int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
@@ -500,37 +556,17 @@ extern "C" int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshot
     std::vector<ShotPlan> plans;
     SAM2D_TRY(make_plans(ctx, shots, nshots, windows, plans));
     ctx->quirk_valid = false;
-    std::vector<const float *> adj_ptr(nshots);
-    std::vector<float *> seis_ptr(nshots);
-    size_t off = 0;
-    for (int s = 0; s < nshots; s++) {
-        adj_ptr[s] = adj + off;
-        seis_ptr[s] = seis ? seis + off : nullptr;
-        off += (size_t)shots[s].nrec * ta.real_nt;
-    }
+    ctx->adj->held = false;                  // bufs are reused: held checkpoints go
+    const std::vector<const float *> adj_ptr = per_shot(adj, shots, nshots, ta.real_nt);
+    const std::vector<float *> seis_ptr = per_shot(seis, shots, nshots, ta.real_nt);
     // Shots per batch: the option, else the fewest batches of equal shot
-    // counts whose ring, checkpoints and adjoint state fit the HBM budget:
-    // "hbm_budget_gb", or by default the device memory that is free or held
-    // by this context's reusable buffers, less a reserve of 5 % of the device
-    // (at least 4 GiB), and at most 90 % of the device.  The 288 GB of an
-    // MI355X hold a 32-shot C4 batch at k = 100 (217 GB) in one piece.
+    // counts whose ring, checkpoints and adjoint state fit the budget of
+    // gradient_budget.  The 288 GB of an MI355X hold a 32-shot C4 batch at
+    // k = 100 (189 GB) in one piece.
     int batch = ctx->max_batch > 0 ? ctx->max_batch : nshots;
     if (ctx->max_batch <= 0 && nshots > 0) {
-        size_t budget = (size_t)ctx->hbm_budget_gb << 30;
-        if (ctx->hbm_budget_gb <= 0) {
-            size_t free_b = 0, total_b = 0;
-            HIP_TRY(hipSetDevice(ctx->device));
-            HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-            size_t held = 0;
-            for (auto &b : ctx->bufs)
-                if (b) held += b->cap_state * sizeof(float);
-            for (auto &b : ctx->adj->bufs)
-                if (b)
-                    for (size_t c : b->cap) held += c;
-            const size_t reserve = std::max<size_t>((size_t)4 << 30, total_b / 20);
-            const size_t avail = free_b + held > reserve ? free_b + held - reserve : 0;
-            budget = std::min(avail, total_b / 10 * 9);
-        }
+        size_t budget = 0;
+        SAM2D_TRY(gradient_budget(ctx, &budget));
         auto fits = [&](int b) {
             for (int s0 = 0; s0 < nshots; s0 += b) {
                 size_t used = 0;
@@ -554,6 +590,73 @@ extern "C" int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshot
         SAM2D_TRY(gradient_batch(ctx, sub, adj_ptr.data() + s0, k,
                                  seis ? seis_ptr.data() + s0 : nullptr, gv, gr));
     }
+    for (size_t i = 0; i < gv.size(); i++) { g_vp[i] = (float)gv[i]; g_rho[i] = (float)gr[i]; }
+    ctx->stats.wall_ms = now_ms() - t0;
+    return SAM2D_OK;
+}
+
+extern "C" int sam2d_forward_checkpointed(sam2d_ctx *ctx, const sam2d_shot *shots, int nshots,
+                                          const int *windows, int ckpt_k, float *seis)
+{
+    clear_error();
+    if (!ctx) return fail(SAM2D_ERR_PARAM, "ctx is NULL");
+    const TimeAxis &ta = ctx->ta;
+    size_t total = 0;
+    for (int s = 0; s < nshots; s++) total += (size_t)shots[s].nrec * ta.real_nt;
+    if (total > 0 && !seis) return fail(SAM2D_ERR_PARAM, "seis is NULL");
+    const double t0 = now_ms();
+    ctx->stats = sam2d_stats();
+    if (!ctx->adj) ctx->adj = new AdjointState();
+    AdjointState &A = *ctx->adj;
+    A.held = false;
+    const int k = ckpt_k < 1 ? 1 : (ckpt_k > ta.fd_nt ? ta.fd_nt : ckpt_k);
+    std::vector<ShotPlan> plans;
+    SAM2D_TRY(make_plans(ctx, shots, nshots, windows, plans));
+    ctx->quirk_valid = false;
+    // Every shot stays resident until the backward pass: one batch.
+    size_t budget = 0, used = 0;
+    SAM2D_TRY(gradient_budget(ctx, &budget));
+    for (auto &p : plans) used += shot_bytes(p, ta.fd_nt, k);
+    if (used > budget)
+        return fail(SAM2D_ERR_MEMORY, "checkpoints, ring and adjoint state of %d shots need %zu "
+                    "MB, over the %zu MB budget (sam2d_gradient runs them in batches)", nshots,
+                    used >> 20, budget >> 20);
+    const std::vector<float *> seis_ptr = per_shot(seis, shots, nshots, ta.real_nt);
+    std::vector<double> none;
+    SAM2D_TRY(gradient_batch(ctx, plans, nullptr, k, seis_ptr.data(), none, none, PH_SWEEP));
+    A.held_plans = plans;
+    A.held_k = k;
+    A.held = true;
+    ctx->stats.wall_ms = now_ms() - t0;
+    return SAM2D_OK;
+}
+
+extern "C" int sam2d_gradient_checkpointed(sam2d_ctx *ctx, const float *adj, float *g_vp,
+                                           float *g_rho)
+{
+    clear_error();
+    if (!ctx) return fail(SAM2D_ERR_PARAM, "ctx is NULL");
+    if (!g_vp || !g_rho) return fail(SAM2D_ERR_PARAM, "gradient outputs are NULL");
+    if (!ctx->adj || !ctx->adj->held)
+        return fail(SAM2D_ERR_PARAM, "no checkpoints held: call sam2d_forward_checkpointed first "
+                    "(a sam2d_gradient call in between drops them)");
+    AdjointState &A = *ctx->adj;
+    const TimeAxis &ta = ctx->ta;
+    size_t total = 0;
+    for (auto &p : A.held_plans) total += (size_t)p.ng * ta.real_nt;
+    if (total > 0 && !adj) return fail(SAM2D_ERR_PARAM, "adj is NULL");
+    const double t0 = now_ms();
+    ctx->stats = sam2d_stats();
+    ctx->quirk_valid = false;
+    std::vector<const float *> adj_ptr(A.held_plans.size());
+    size_t off = 0;
+    for (size_t s = 0; s < A.held_plans.size(); s++) {
+        adj_ptr[s] = adj + off;
+        off += (size_t)A.held_plans[s].ng * ta.real_nt;
+    }
+    std::vector<double> gv((size_t)ctx->grid.nx * ctx->grid.nz, 0.0), gr(gv.size(), 0.0);
+    std::vector<ShotPlan> plans = A.held_plans;
+    SAM2D_TRY(gradient_batch(ctx, plans, adj_ptr.data(), A.held_k, nullptr, gv, gr, PH_BACKWARD));
     for (size_t i = 0; i < gv.size(); i++) { g_vp[i] = (float)gv[i]; g_rho[i] = (float)gr[i]; }
     ctx->stats.wall_ms = now_ms() - t0;
     return SAM2D_OK;

@@ -197,6 +197,51 @@ def test_gradient_forward_seismograms_match_forward():
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("windows", [None, [(5, 40), (30, 45)]])
+def test_forward_checkpointed_and_backward_equal_the_gradient_bitwise(windows):
+    # sam2d_forward_checkpointed + sam2d_gradient_checkpointed are the two
+    # halves of sam2d_gradient: the same seismograms as the forward call and
+    # the same gradient bits, the backward repeatable with other sources and
+    # unaffected by modelling calls in between; a sam2d_gradient call drops
+    # the checkpoints.
+    grid, vp, rho, shots = _setup(nx=80, T=0.15)
+    if windows is not None:
+        shots = []
+        for wx0, w in windows:
+            rec = np.stack([(wx0 + np.arange(w)) * 10.0 + 5, np.full(w, 30.0)], 1)
+            shots.append(sam.Shot((wx0 + w // 2) * 10.0 + 5, 60.0, rec))
+    rng = np.random.default_rng(8)
+    with sam.Modeller(grid, vp, rho) as m:
+        ref_seis = m.forward(shots) if windows is None else m.forward_windows(shots, windows)
+        adj1 = [rng.standard_normal(f.shape).astype(np.float32) for f in ref_seis]
+        adj2 = [rng.standard_normal(f.shape).astype(np.float32) for f in ref_seis]
+        ref1 = m.gradient(shots, adj1, ckpt_k=13, windows=windows)
+        ref2 = m.gradient(shots, adj2, ckpt_k=13, windows=windows)
+        seis = m.forward_checkpointed(shots, ckpt_k=13, windows=windows)
+        for a, b in zip(seis, ref_seis):
+            assert np.array_equal(a.view(np.int32), b.view(np.int32))
+        g1 = m.gradient_checkpointed(adj1)
+        m.forward(shots)                      # other modelling in between
+        g2 = m.gradient_checkpointed(adj2)
+        for g, ref in ((g1, ref1), (g2, ref2)):
+            assert np.array_equal(g[0].view(np.int32), ref[0].view(np.int32))
+            assert np.array_equal(g[1].view(np.int32), ref[1].view(np.int32))
+        m.gradient(shots, adj1, ckpt_k=13, windows=windows)
+        with pytest.raises(sam.SAM2DError):
+            m.gradient_checkpointed(adj1)
+
+
+def test_forward_checkpointed_refuses_shots_beyond_the_budget():
+    # 8.1 M cells, 50 checkpoints of 16 B per cell: about 8 GB against 1 GiB
+    big = sam.Grid(4000, 2000, 10.0, 10.0, 0.5, 0.001, 0.002, 25.0, 8)
+    vpb = np.full((4000, 2000), 2000.0, np.float32)
+    with sam.Modeller(big, vpb, vpb) as m:
+        m.set_option("hbm_budget_gb", 1)
+        rec = np.stack([np.arange(100) * 10.0, np.full(100, 30.0)], 1)
+        with pytest.raises(sam.SAM2DError):
+            m.forward_checkpointed([sam.Shot(500.0, 50.0, rec)], ckpt_k=10)
+
+
 def test_windowed_gradient_vs_oracle_on_cropped_models(oracle):
     grid, vp, rho, _ = _setup(nx=80, T=0.15)
     f32 = np.float32
