@@ -149,23 +149,28 @@ def cpu_baseline(vp, rho, seconds_target=15.0):
                            f"5100x5100 padded grid, {th} OpenMP threads, setup excluded"}, **cpus)
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, config):
     """HBM bytes per launch of the dominant kernel from rocprofv3 --pmc, as
-    recorded in profiles/<kernel>_config1_pmc.txt ("traffic_bytes_per_launch:
-    <n>" and the variant line "profiled: <variant> chunk <c> order <o>")."""
+    recorded in profiles/<kernel>_config1_pmc.txt, one line per profiled
+    configuration: "profiled: <variant> chunk <c> order <o>
+    traffic_bytes_per_launch <n>".  Returns (bytes, the configuration they
+    were measured at): the run's own configuration when it was profiled, else
+    the first profiled one."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_config1_pmc.txt")
     if not os.path.exists(path):
         return None, None
-    traffic, cfg = None, None
+    found = []
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith("profiled:") and "traffic_bytes_per_launch" in line:
+            cfg, n = line.split(":", 1)[1].split("traffic_bytes_per_launch")
             try:
-                traffic = float(line.split(":", 1)[1])
+                found.append((cfg.strip(), float(n)))
             except ValueError:
                 pass
-        elif line.startswith("profiled:"):
-            cfg = line.split(":", 1)[1].strip()
-    return traffic, cfg
+    for cfg, n in found:
+        if cfg == config:
+            return n, cfg
+    return (found[0][1], found[0][0]) if found else (None, None)
 
 
 # --------------------------------------------------------------- config 1
@@ -236,8 +241,8 @@ def bench_plain(args, sam):
     hbm_bytes = bpc * cells
     achieved = hbm_bytes / (launch_ms * 1e-3) / 1e9
     kname = "k_step" if spl == 1 else "k_step%d" % spl
-    traffic, traffic_cfg = measured_traffic(kname)
     this_cfg = f"{variant} chunk {chunk} order {tile_order}"
+    traffic, traffic_cfg = measured_traffic(kname, this_cfg)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": kname, "steps_per_launch": spl,
