@@ -11,11 +11,12 @@
 //      then run the adjoint steps of the segment in reverse, two per pass
 //      where every shot has a residual band, step n reading ring slot
 //      n - m*k;
-//   3. copy dJ/dkappa, dJ/dbu2, dJ/dbu1 to the host and chain them in FP64 to
-//      dJ/dvp and dJ/drho through the buoyancy staggering, kappa = vp^2 rho and
-//      the sponge padding (the transpose of cpu_stagger,
+//   3. chain dJ/dkappa, dJ/dbu2, dJ/dbu1 on the device, in FP64, to dJ/dvp
+//      and dJ/drho through the buoyancy staggering, kappa = vp^2 rho and the
+//      sponge padding (the transpose of cpu_stagger,
 //      src/src_cpu/cpu_modelling_kernels.cpp:64-121, and expand,
-//      src/src_cpu/cpumodelling.cpp:39-60).
+//      src/src_cpu/cpumodelling.cpp:39-60), shot after shot into the model's
+//      gradients, which reach the host once per call.
 #include "sam2d_internal.h"
 
 #include <algorithm>
@@ -58,7 +59,16 @@ struct AdjointState {
     bool held = false;
     std::vector<ShotPlan> held_plans;
     int held_k = 0;
-    ~AdjointState() { dev_free(d_desc); dev_free(d_desc2); dev_free(d_fwd); dev_free(d_ring); }
+    // The model (vp then rho) and its FP64 gradients (dJ/dvp then dJ/drho)
+    // on the device, nx*nz each.
+    float *d_model = nullptr;
+    double *d_grad = nullptr;
+    size_t cap_model = 0, cap_grad = 0;
+    ~AdjointState()
+    {
+        dev_free(d_desc); dev_free(d_desc2); dev_free(d_fwd); dev_free(d_ring);
+        dev_free(d_model); dev_free(d_grad);
+    }
 };
 
 static int grow(void **p, size_t &cap, size_t need)
@@ -81,47 +91,6 @@ static int rec_sample(const TimeAxis &ta, int n)
 }
 
 // Host chain rule of one shot's padded-grid gradients into the model grid.
-static void chain_to_model(const sam2d_ctx *ctx, const DomainHost &h, const std::vector<float> &gk,
-                           const std::vector<float> &gbx, const std::vector<float> &gbz,
-                           std::vector<double> &g_vp, std::vector<double> &g_rho)
-{
-    const int nnz = h.nnz, nnx = h.nnx, P = ctx->grid.npml, nz = ctx->grid.nz;
-    const size_t N = (size_t)nnz * nnx;
-    std::vector<double> b1(gbz.begin(), gbz.end()), b2(gbx.begin(), gbx.end());
-    // bu2's last column is bu1's column nnx-2.
-    for (int iz = 0; iz < nnz; iz++) b1[(size_t)(nnx - 2) * nnz + iz] += b2[(size_t)(nnx - 1) * nnz + iz];
-    // bu1 = (b[iz] + b[iz+1]) / 2 except the last row (a constant 0);
-    // bu2 = (b[ix] + b[ix+1]) / 2 except the last column (handled above).
-    std::vector<double> gb(N, 0.0);
-    for (int ix = 0; ix < nnx; ix++)
-        for (int iz = 0; iz + 1 < nnz; iz++) {
-            const size_t id = (size_t)ix * nnz + iz;
-            gb[id] += 0.5 * b1[id];
-            gb[id + 1] += 0.5 * b1[id];
-        }
-    for (int ix = 0; ix + 1 < nnx; ix++)
-        for (int iz = 0; iz < nnz; iz++) {
-            const size_t id = (size_t)ix * nnz + iz;
-            gb[id] += 0.5 * b2[id];
-            gb[id + nnz] += 0.5 * b2[id];
-        }
-    for (int ix = 0; ix < nnx; ix++) {
-        int mx = ix - P;
-        mx = mx < 0 ? 0 : (mx >= h.w ? h.w - 1 : mx);
-        const size_t col = (size_t)(h.wx0 + mx) * nz;
-        for (int iz = 0; iz < nnz; iz++) {
-            int mz = iz - P;
-            mz = mz < 0 ? 0 : (mz >= nz ? nz - 1 : mz);
-            const size_t id = (size_t)ix * nnz + iz;
-            const double v = ctx->vp[col + mz], r = ctx->rho[col + mz];
-            g_vp[col + mz] += 2.0 * v * r * (double)gk[id];
-            double gr = v * v * (double)gk[id];
-            if (ctx->rho[col + mz] > 0.0f) gr -= gb[id] / (r * r);
-            g_rho[col + mz] += gr;
-        }
-    }
-}
-
 // Device bytes one shot of `plan` needs for a gradient with checkpoint
 // interval k: two state slots, k ring triplets, the checkpoints, the adjoint
 // state with its accumulators and the adjoint's coefficient cells.
@@ -155,8 +124,7 @@ enum { PH_SWEEP = 1, PH_BACKWARD = 2 };
 // pass without the sweep reads the checkpoints an earlier sweep of the same
 // plans and k left in the batch's buffers.
 static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const float *const *adj,
-                          int k, float *const *seis_out, std::vector<double> &g_vp,
-                          std::vector<double> &g_rho, int phases = PH_SWEEP | PH_BACKWARD)
+                          int k, float *const *seis_out, int phases = PH_SWEEP | PH_BACKWARD)
 {
     const bool backward = phases & PH_BACKWARD;
     AdjointState &A = *ctx->adj;
@@ -455,6 +423,15 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
             lam ^= 1;
         }
     }
+    // 3. the chain rule, shot after shot
+    const size_t nm = (size_t)ctx->grid.nx * ctx->grid.nz;
+    for (int s = 0; s < ns && backward; s++) {
+        const DomainHost &h = plans[s].dom->host;
+        const size_t N = (size_t)h.nnx * h.nnz;
+        HIP_TRY(launch_chain(A.bufs[s]->lam + 8 * N, h.nnz, h.nnx, ctx->grid.npml, h.wx0, h.w,
+                             ctx->grid.nz, A.d_model, A.d_model + nm, A.d_grad, A.d_grad + nm,
+                             ctx->stream));
+    }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev1));
     float ms = 0.0f;
@@ -466,26 +443,38 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     ctx->stats.variant = variant;
     ctx->stats.steps_per_launch = 1;
 
-    // 3. gradients to the host, chain rule
     for (int s = 0; s < ns; s++) {
         const DomainHost &h = plans[s].dom->host;
-        const size_t N = (size_t)h.nnx * h.nnz;
-        ctx->stats.cell_updates += (double)N * ta.fd_nt;
+        ctx->stats.cell_updates += (double)h.nnx * h.nnz * ta.fd_nt;
         if (want_seis && plans[s].ng)
             HIP_TRY(hipMemcpy(seis_out[s], plans[s].seis_dev,
                               (size_t)plans[s].ng * ta.real_nt * sizeof(float), hipMemcpyDeviceToHost));
-        if (!backward) continue;
-        std::vector<float> g(3 * N);
-        HIP_TRY(hipMemcpy(g.data(), A.bufs[s]->lam + 8 * N, 3 * N * sizeof(float),
-                          hipMemcpyDeviceToHost));
-        std::vector<float> gk(N), gbx(N), gbz(N);
-        for (size_t i = 0; i < N; i++) {
-            gk[i] = g[3 * i];
-            gbx[i] = g[3 * i + 1];
-            gbz[i] = g[3 * i + 2];
-        }
-        chain_to_model(ctx, h, gk, gbx, gbz, g_vp, g_rho);
     }
+    return SAM2D_OK;
+}
+
+// The model on the device and its gradients cleared, before a call's batches.
+static int begin_gradients(sam2d_ctx *ctx)
+{
+    AdjointState &A = *ctx->adj;
+    const size_t nm = (size_t)ctx->grid.nx * ctx->grid.nz;
+    SAM2D_TRY(grow((void **)&A.d_model, A.cap_model, 2 * nm * sizeof(float)));
+    SAM2D_TRY(grow((void **)&A.d_grad, A.cap_grad, 2 * nm * sizeof(double)));
+    SAM2D_TRY(upload(A.d_model, ctx->vp.data(), nm * sizeof(float), ctx->stream));
+    SAM2D_TRY(upload(A.d_model + nm, ctx->rho.data(), nm * sizeof(float), ctx->stream));
+    HIP_TRY(launch_fill((float *)A.d_grad, 4 * nm, 0.0f, ctx->stream));   // +0.0 in FP64
+    return SAM2D_OK;
+}
+
+// The gradients to the caller's FP32 arrays.
+static int end_gradients(sam2d_ctx *ctx, float *g_vp, float *g_rho)
+{
+    const size_t nm = (size_t)ctx->grid.nx * ctx->grid.nz;
+    std::vector<double> g(2 * nm);
+    HIP_TRY(hipMemcpyAsync(g.data(), ctx->adj->d_grad, 2 * nm * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < nm; i++) { g_vp[i] = (float)g[i]; g_rho[i] = (float)g[nm + i]; }
     return SAM2D_OK;
 }
 
@@ -583,14 +572,14 @@ extern "C" int sam2d_gradient(sam2d_ctx *ctx, const sam2d_shot *shots, int nshot
             batch = (nshots + nb - 1) / nb;
         }
     }
-    std::vector<double> gv((size_t)ctx->grid.nx * ctx->grid.nz, 0.0), gr(gv.size(), 0.0);
+    SAM2D_TRY(begin_gradients(ctx));
     for (int s0 = 0; s0 < nshots; s0 += batch) {
         const int s1 = std::min(nshots, s0 + batch);
         std::vector<ShotPlan> sub(plans.begin() + s0, plans.begin() + s1);
         SAM2D_TRY(gradient_batch(ctx, sub, adj_ptr.data() + s0, k,
-                                 seis ? seis_ptr.data() + s0 : nullptr, gv, gr));
+                                 seis ? seis_ptr.data() + s0 : nullptr));
     }
-    for (size_t i = 0; i < gv.size(); i++) { g_vp[i] = (float)gv[i]; g_rho[i] = (float)gr[i]; }
+    SAM2D_TRY(end_gradients(ctx, g_vp, g_rho));
     ctx->stats.wall_ms = now_ms() - t0;
     return SAM2D_OK;
 }
@@ -622,8 +611,7 @@ extern "C" int sam2d_forward_checkpointed(sam2d_ctx *ctx, const sam2d_shot *shot
                     "MB, over the %zu MB budget (sam2d_gradient runs them in batches)", nshots,
                     used >> 20, budget >> 20);
     const std::vector<float *> seis_ptr = per_shot(seis, shots, nshots, ta.real_nt);
-    std::vector<double> none;
-    SAM2D_TRY(gradient_batch(ctx, plans, nullptr, k, seis_ptr.data(), none, none, PH_SWEEP));
+    SAM2D_TRY(gradient_batch(ctx, plans, nullptr, k, seis_ptr.data(), PH_SWEEP));
     A.held_plans = plans;
     A.held_k = k;
     A.held = true;
@@ -654,10 +642,10 @@ extern "C" int sam2d_gradient_checkpointed(sam2d_ctx *ctx, const float *adj, flo
         adj_ptr[s] = adj + off;
         off += (size_t)A.held_plans[s].ng * ta.real_nt;
     }
-    std::vector<double> gv((size_t)ctx->grid.nx * ctx->grid.nz, 0.0), gr(gv.size(), 0.0);
     std::vector<ShotPlan> plans = A.held_plans;
-    SAM2D_TRY(gradient_batch(ctx, plans, adj_ptr.data(), A.held_k, nullptr, gv, gr, PH_BACKWARD));
-    for (size_t i = 0; i < gv.size(); i++) { g_vp[i] = (float)gv[i]; g_rho[i] = (float)gr[i]; }
+    SAM2D_TRY(begin_gradients(ctx));
+    SAM2D_TRY(gradient_batch(ctx, plans, adj_ptr.data(), A.held_k, nullptr, PH_BACKWARD));
+    SAM2D_TRY(end_gradients(ctx, g_vp, g_rho));
     ctx->stats.wall_ms = now_ms() - t0;
     return SAM2D_OK;
 }

@@ -920,4 +920,63 @@ hipError_t launch_inject(const AdjDev *d_adj, int nshots, int max_cells, int lam
     return hipGetLastError();
 }
 
+// dJ/dvp and dJ/drho of one shot's padded domain added into the model's FP64
+// gradients, one thread per model cell of the window [wx0, wx0 + w): the
+// transpose of expand (cpumodelling.cpp:39-60, sponge cells copy the nearest
+// model cell), of cpu_stagger (cpu_modelling_kernels.cpp:64-121: bu1 is the
+// mean of b at rows iz, iz+1 except the last row, bu2 that of b at columns
+// ix, ix+1, its last column a copy of bu1's column nnx-2) and of kappa =
+// rho vp^2, b = 1/rho.  A model cell takes the padded cells that copy it in
+// ascending (ix, iz) order and each term is rounded as written, so the sums
+// are those of the same loop run on the host, and shots are added one launch
+// after another in a fixed order.
+__global__ void k_chain(const float *__restrict__ acc, int nnz, int nnx, int npml, int wx0, int w,
+                        int nz, const float *__restrict__ vp, const float *__restrict__ rho,
+                        double *__restrict__ g_vp, double *__restrict__ g_rho)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= w * nz) return;
+    const int mx = t / nz, mz = t - mx * nz;
+    const size_t col = (size_t)(wx0 + mx) * nz + mz;
+    const int ixl = mx == 0 ? 0 : npml + mx, ixh = mx == w - 1 ? nnx - 1 : npml + mx;
+    const int izl = mz == 0 ? 0 : npml + mz, izh = mz == nz - 1 ? nnz - 1 : npml + mz;
+    const double v = vp[col], r = rho[col];
+    const bool pos = rho[col] > 0.0f;
+    double gv = g_vp[col], gr = g_rho[col];
+    // b1 = dJ/dbu1 in FP64, with bu2's last column folded into column nnx-2
+    auto b1 = [&](int ix, size_t id) -> double {
+        double x = acc[3 * id + 2];
+        if (ix == nnx - 2) x += (double)acc[3 * (id + nnz) + 1];
+        return x;
+    };
+    auto b2 = [&](size_t id) -> double { return acc[3 * id + 1]; };
+    for (int ix = ixl; ix <= ixh; ix++)
+        for (int iz = izl; iz <= izh; iz++) {
+            const size_t id = (size_t)ix * nnz + iz;
+            const double gk = acc[3 * id];
+            double gb = 0.0;
+            if (iz >= 1) gb += 0.5 * b1(ix, id - 1);
+            if (iz < nnz - 1) gb += 0.5 * b1(ix, id);
+            if (ix >= 1) gb += 0.5 * b2(id - nnz);
+            if (ix < nnx - 1) gb += 0.5 * b2(id);
+            gv += 2.0 * v * r * gk;
+            double g = v * v * gk;
+            if (pos) g -= gb / (r * r);
+            gr += g;
+        }
+    g_vp[col] = gv;
+    g_rho[col] = gr;
+}
+
+hipError_t launch_chain(const float *acc, int nnz, int nnx, int npml, int wx0, int w, int nz,
+                        const float *vp, const float *rho, double *g_vp, double *g_rho,
+                        hipStream_t stream)
+{
+    const size_t n = (size_t)w * nz;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chain, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, acc, nnz,
+                       nnx, npml, wx0, w, nz, vp, rho, g_vp, g_rho);
+    return hipGetLastError();
+}
+
 }  // namespace sam2d

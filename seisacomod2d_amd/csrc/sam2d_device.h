@@ -143,6 +143,14 @@ hipError_t launch_band(const int *bmap, const int *rptr, const int *rid, const f
 hipError_t launch_inject(const AdjDev *d_adj, int nshots, int max_cells, int lam, int rec_k,
                          hipStream_t stream);
 
+// dJ/dvp, dJ/drho of one shot added into the model's FP64 gradients
+// (nx x nz, x-major): acc holds the shot's (dJ/dkappa, dJ/dbu2, dJ/dbu1)
+// cells on its padded domain of nnx x nnz cells, built from model columns
+// [wx0, wx0 + w) with npml sponge cells on each side; vp, rho the model.
+hipError_t launch_chain(const float *acc, int nnz, int nnx, int npml, int wx0, int w, int nz,
+                        const float *vp, const float *rho, double *g_vp, double *g_rho,
+                        hipStream_t stream);
+
 int num_variants();
 void variant_shape(int v, int *waves, int *chunk);
 // The same plus the operand columns in flight (prefetch depth).
