@@ -263,7 +263,8 @@ def fp32_forward(fields, fd_nt, nts, amp, src_cell, gcells, dt_factor, real_nt, 
     fields: KX, KZ, BX, BZ (nnx, nnz) and prof_x (4, nnx), prof_z (4, nnz) as
     the library builds them; amp: source increments; src_cell, gcells: padded
     cell indices ix*nnz + iz.  state: (px, pz, vx, vz) to start from (updated
-    in place), default zero.  Returns (seis (ng, real_nt), state)."""
+    in place), default zero, in the kernels' representation (outside the
+    sponge frame px holds p and pz is 0).  Returns (seis (ng, real_nt), state)."""
     KX = _f32(fields["KX"])
     nnx, nnz = KX.shape
     f = {k: _f32(fields[k]) for k in ("KZ", "BX", "BZ", "prof_x", "prof_z")}

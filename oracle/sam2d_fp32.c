@@ -19,6 +19,12 @@
  *   px'     = fma(KX * sx1[ix], Dx- vx, sx0[ix] * px)  (+ the source increment)
  *   pz'     = fma(KZ * sz1[iz], Dz- vz, sz0[iz] * pz)
  *   p       = px' + pz'
+ *
+ * Outside the sponge frame (columns and rows where all four profiles of the
+ * axis are exactly 1, the middle run of such indices, as the library's
+ * ones_range finds it) the kernels keep p itself in the px slot and pz = 0:
+ * there px' = fma(KX, Dx- vx, p) (+ source), pz' = fl(KZ * Dz- vz) and the
+ * stored p = px' + pz'; the same formulas with pz = 0.
  *   Dx+ p   = fma(c0, p[i+1] - p[i], c1 * (p[i+2] - p[i-1]))   (fdoperator421/411)
  *   vx'     = fma(BX * bx1[ix], Dx+ p, bx0[ix] * vx),  vz' likewise
  *
@@ -54,8 +60,35 @@ static inline void f32_dplus(int i, int n, float *c0, float *c1)
  * coefficient, as in the kernels' clamped loads. */
 static inline long f32_cl(long i, long n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
 
+/* The middle run of indices where all four profiles [4][n] are exactly 1
+ * (lo > hi: none). */
+static void f32_ones_range(const float *prof, int n, int *lo, int *hi)
+{
+    const int mid = n / 2;
+    int k, a, b, ok = 1;
+    for (k = 0; k < 4; k++) ok &= prof[(long)k * n + mid] == 1.0f;
+    if (!ok) { *lo = 1; *hi = 0; return; }
+    a = b = mid;
+    for (;;) {
+        if (a == 0) break;
+        ok = 1;
+        for (k = 0; k < 4; k++) ok &= prof[(long)k * n + a - 1] == 1.0f;
+        if (!ok) break;
+        a--;
+    }
+    for (;;) {
+        if (b == n - 1) break;
+        ok = 1;
+        for (k = 0; k < 4; k++) ok &= prof[(long)k * n + b + 1] == 1.0f;
+        if (!ok) break;
+        b++;
+    }
+    *lo = a; *hi = b;
+}
+
 /* fd_nt steps of one shot from the state (px, pz, vx, vz), each nnx*nnz
- * floats, x-major, updated in place.  prof_x = [damp2a, damp2a1, damp2b,
+ * floats, x-major, updated in place, in the kernels' representation: outside
+ * the sponge frame px holds p and pz is 0.  prof_x = [damp2a, damp2a1, damp2b,
  * damp2b1][nnx], prof_z = [damp1a, damp1a1, damp1b, damp1b1][nnz]; amp[nts]
  * source increments at cell src (injected while step <= nts, the increment
  * past the wavelet's end being 0); seis[ng][real_nt] samples p at the cells
@@ -70,6 +103,9 @@ int f32_forward(int nnx, int nnz, const float *KX, const float *KZ, const float 
     if (!p) return -30;
     const float *dax = prof_x, *skx = prof_x + nnx, *dbx = prof_x + 2 * nnx, *sbx = prof_x + 3 * nnx;
     const float *daz = prof_z, *skz = prof_z + nnz, *dbz = prof_z + 2 * nnz, *sbz = prof_z + 3 * nnz;
+    int xlo, xhi, zlo, zhi;
+    f32_ones_range(prof_x, nnx, &xlo, &xhi);
+    f32_ones_range(prof_z, nnz, &zlo, &zhi);
     for (int it = 0; it < fd_nt; it++) {
         /* p-phase: Dx- vx, Dz- vz, cpu_pml_p, source, p = px + pz */
         #pragma omp parallel for schedule(static)
@@ -90,9 +126,14 @@ int f32_forward(int nnx, int nnz, const float *KX, const float *KZ, const float 
                 float pxn = fmaf(KX[id] * skx[ix], dvx, dax[ix] * px[id]);
                 const float pzn = fmaf(KZ[id] * skz[iz], dvz, daz[iz] * pz[id]);
                 if (id == src && it <= nts) pxn += it < nts ? amp[it] : 0.0f;
-                px[id] = pxn;
-                pz[id] = pzn;
-                p[id] = pxn + pzn;
+                const float pn = pxn + pzn;
+                if (ix >= xlo && ix <= xhi && iz >= zlo && iz <= zhi) {
+                    px[id] = pn;            /* outside the frame: p, and pz stays 0 */
+                } else {
+                    px[id] = pxn;
+                    pz[id] = pzn;
+                }
+                p[id] = pn;
             }
         }
         /* v-phase: Dx+ p, Dz+ p, cpu_pml_v */

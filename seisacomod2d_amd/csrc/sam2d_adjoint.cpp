@@ -306,13 +306,10 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     // variant and march length, in device buffer *buf (grown as needed).
     auto multistep_desc = [&](int v, int chunk, ShotDev **buf, int *cap, int *tiles,
                               int *groups) -> int {
-        int w2, c2;
-        variant_shape(v, &w2, &c2);
         std::vector<ShotDev> d2(ctx->staged_desc.begin(), ctx->staged_desc.begin() + ns);
         int t2 = 0;
         for (auto &dd : d2) {
-            tile_shape(dd.nnz, dd.nnx, w2, chunk, variant_own_rows(v), dd.ngroups, dd.nchunks);
-            dd.chunk = chunk;
+            shot_tiles(dd, v, chunk);
             dd.tile_begin = t2;
             t2 += dd.ngroups * dd.nchunks;
         }

@@ -194,6 +194,36 @@ void tile_shape(int nnz, int nnx, int waves, int chunk, int own_rows, int &ngrou
     nchunks = (nnx + chunk - 1) / chunk;
 }
 
+// Tiles of a shot for a variant and march length chunk (d.nnz, d.nnx and the
+// x-profile range xin_lo..xin_hi set): z-strip groups and chunk columns.  For
+// a multi-step variant the x-frame goes into chunk columns of its own,
+// [0, x0) and [x1, nnx), with x0 and x1 chosen so that the march of every
+// other tile lies in the kernel's fast range (fast_tile in
+// sam2d_kernelsn.h): those tiles carry no sponge-frame columns, and the
+// frame tiles, whose march takes the general path, are narrow.
+void shot_tiles(ShotDev &d, int variant, int chunk)
+{
+    int waves, vchunk, pf;
+    variant_shape_pf(variant, &waves, &vchunk, &pf);
+    const int T = variant_steps(variant);
+    tile_shape(d.nnz, d.nnx, waves, chunk, variant_own_rows(variant), d.ngroups, d.nchunks);
+    d.chunk = chunk;
+    d.x0 = d.x1 = 0;
+    if (T < 2 || d.xin_lo > d.xin_hi || chunk < 1) return;
+    const int fast_lo = std::max(d.xin_lo + 3 * T - 1, 3 * T);
+    const int fast_hi = std::min(d.xin_hi - 4 - pf, d.nnx - 5);
+    const int x0 = fast_lo + 3 * T - 2;          // first march column = fast_lo
+    const int xmax = fast_hi - 3 * T + 5;        // last group's first column = fast_hi
+    if (xmax - x0 < chunk) return;
+    const int m = (xmax - x0) / chunk;
+    int w = (xmax - x0) - m * chunk;
+    w -= (w + 6 * T - 3) & 3;                    // a march of whole four-column groups
+    if (w < 0) w = 0;
+    d.x0 = x0;
+    d.x1 = x0 + m * chunk + w;
+    d.nchunks = 2 + m + (w > 0 ? 1 : 0);
+}
+
 int pick_variant(sam2d_ctx *ctx)
 {
     if (ctx->variant >= 0) return ctx->variant;
@@ -235,8 +265,6 @@ int tile_chunk(const sam2d_ctx *ctx, int variant)
 void describe_shot(const ShotPlan &p, int variant, int chunk, int tile_begin, int nts, int real_nt,
                    ShotDev &d)
 {
-    int waves, vchunk;
-    variant_shape(variant, &waves, &vchunk);
     const DeviceDomain *dom = p.dom;
     const int nnz = dom->host.nnz, nnx = dom->host.nnx;
     int rlo = 1, rhi = 0;
@@ -247,12 +275,12 @@ void describe_shot(const ShotPlan &p, int variant, int chunk, int tile_begin, in
     }
     std::memset(&d, 0, sizeof d);
     d.nnz = nnz; d.nnx = nnx;
-    tile_shape(nnz, nnx, waves, chunk, variant_own_rows(variant), d.ngroups, d.nchunks);
-    d.chunk = chunk;
+    ones_range(dom->host.prof_x, nnx, d.xin_lo, d.xin_hi);
+    ones_range(dom->host.prof_z, nnz, d.zin_lo, d.zin_hi);
+    shot_tiles(d, variant, chunk);
     d.tile_begin = tile_begin;
     d.src_x = p.src_x; d.src_z = p.src_z;
     d.nts = nts; d.ng = p.ng; d.real_nt = real_nt;
-    ones_range(dom->host.prof_x, nnx, d.xin_lo, d.xin_hi);
     d.rrow_lo = rlo; d.rrow_hi = rhi; d.band_h = p.ng > 0 ? rhi - rlo + 1 : 0;
     d.KX = dom->KX; d.KZ = dom->KZ; d.BX = dom->BX; d.BZ = dom->BZ; d.B = dom->B;
     d.dtdx = (float)dom->host.dtdx; d.dtdz = (float)dom->host.dtdz;
@@ -260,8 +288,8 @@ void describe_shot(const ShotPlan &p, int variant, int chunk, int tile_begin, in
     d.ncell = (size_t)nnz * nnx;
 }
 
-// 32-bit byte offsets of the multi-step kernels: 16 N bytes of
-// cell-interleaved state, 4 N per field.
+// 32-bit byte offsets of the multi-step kernels: a state slot spans 16 N
+// bytes (the 12-byte cells, then pz), a coefficient field 4 N.
 int check_multistep_size(const sam2d_ctx *ctx, int variant, size_t ncell)
 {
     if (variant_steps(variant) > 1 && ncell >= ((size_t)1 << 28))
@@ -295,7 +323,9 @@ int stage_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, int variant, int *
         describe_shot(p, variant, chunk, tiles, nts, real_nt, d);
         tiles += d.ngroups * d.nchunks;
         SAM2D_TRY(b.reserve_band((size_t)2 * BAND_SLICES * d.band_h * nnx));
-        HIP_TRY(launch_fill(b.slot(0), 4 * N, 0.0f, ctx->stream));
+        // Every slot starts at zero: the kernels write pz only in the sponge
+        // frame and read it as 0 elsewhere.
+        HIP_TRY(launch_fill(b.slot(0), (size_t)nslots * 4 * N, 0.0f, ctx->stream));
         if (p.ng) HIP_TRY(launch_fill(b.seis, (size_t)p.ng * real_nt, 0.0f, ctx->stream));
         d.amp = b.amp;
         d.state = b.state;
@@ -518,8 +548,10 @@ static int forward_reset_quirk(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, flo
                                    hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             const int nnz = one[0].dom->host.nnz;
-            // cells 0 .. nnz-1, four fields each
-            std::fill(keep.begin(), keep.begin() + 4 * (size_t)nnz, 0.0f);
+            // cells 0 .. nnz-1 (column 0, in the sponge frame): their
+            // (px, vx, vz) and their pz
+            std::fill(keep.begin(), keep.begin() + 3 * (size_t)nnz, 0.0f);
+            std::fill(keep.begin() + 3 * N, keep.begin() + 3 * N + nnz, 0.0f);
         }
         int ntiles = 0;
         ctx->quirk_valid = false;

@@ -25,11 +25,15 @@ struct ShotDev {
     int nnz, nnx;          // padded domain of this shot
     int ngroups, nchunks;  // tiles: z-strip groups x x-chunks
     int chunk;             // columns per x-chunk
+    int x0, x1;            // x0 < x1: chunk column 0 is [0, x0), the last [x1, nnx),
+                           //   those between are chunk-wide from x0 (the last of them
+                           //   ends at x1); otherwise chunk-wide from 0 (tile_columns)
     int tile_begin;        // first flat tile id of this shot
     int src_x, src_z;      // source cell (padded indices)
     int nts;               // wavelet length in steps
     int ng, real_nt;       // receivers, samples per trace
     int xin_lo, xin_hi;    // columns where all four x-profiles are exactly 1
+    int zin_lo, zin_hi;    // rows where all four z-profiles are exactly 1
     int rrow_lo, rrow_hi;  // padded rows of the receivers (lo > hi: none)
     int band_h;            // rrow_hi - rrow_lo + 1 (0: no receivers)
     const float *amp;      // [nts] source increments (fd_dt*w*kappa_s/(dx*dz))
@@ -39,8 +43,8 @@ struct ShotDev {
     float dtdx, dtdz;      // fd_dt/dx, fd_dt/dz (float products, as the reference forms them)
     const float *prof_x;   // [4][nnx]: damp2a, damp2a1, damp2b, damp2b1
     const float *prof_z;   // [4][nnz]: damp1a, damp1a1, damp1b, damp1b1
-    float *state;          // state slots, cell-interleaved: px, pz, vx, vz of cell i
-    size_t ncell;          //   of slot s at state + 4*(s*ncell + i)
+    float *state;          // state slots of 4*ncell floats (slot_cells, slot_pz)
+    size_t ncell;
     float *ring;           // recomputation only: forward ring of triplets, slot m
                            //   (segment step m) at ring + 3*m*ncell, cell i holding
                            //   p after step m and vx, vz before it at [3i .. 3i+2]
@@ -50,13 +54,42 @@ struct ShotDev {
                            // rows after the intermediate steps of a launch
 };
 
-// State slot s: px, pz, vx, vz of cell i at [4i .. 4i+3].  Per column a wave
-// reads and writes one contiguous kilobyte of state instead of four 256-byte
-// runs ncell*4 bytes apart, which the memory system streams about 18 % faster
-// for the kernels' tile pattern on MI355X.
+// State slot s holds 4*ncell floats: the cells, then the frame's pz.
+//  * cells: (P, vx, vz) of cell i at [3i .. 3i+2].  Inside the sponge frame
+//    P is the split field px; outside it (columns xin_lo..xin_hi and rows
+//    zin_lo..zin_hi, where all eight profile factors are exactly 1) the split
+//    is never needed, since px and pz are only ever used as their sum there,
+//    and P is the pressure p = px + pz itself.
+//  * pz of cell i at [3*ncell + i], read and written only in the frame; zero
+//    elsewhere.
+// Per column a wave reads and writes one contiguous 768-byte run of cells
+// (the frame adds a masked 256-byte run); outside the frame a cell moves
+// 12 bytes each way instead of 16.
 __host__ __device__ inline float *slot_cells(const ShotDev &d, int slot)
 {
     return d.state + (size_t)slot * 4 * d.ncell;
+}
+__host__ __device__ inline float *slot_pz(const ShotDev &d, int slot)
+{
+    return d.state + (size_t)slot * 4 * d.ncell + 3 * d.ncell;
+}
+// Columns [xa, xb) of chunk column k of a shot's tiles.
+__host__ __device__ inline void tile_columns(const ShotDev &d, int k, int &xa, int &xb)
+{
+    if (d.x0 < d.x1) {
+        if (k == 0) { xa = 0; xb = d.x0; }
+        else if (k == d.nchunks - 1) { xa = d.x1; xb = d.nnx; }
+        else { xa = d.x0 + (k - 1) * d.chunk; xb = xa + d.chunk < d.x1 ? xa + d.chunk : d.x1; }
+    } else {
+        xa = k * d.chunk;
+        xb = xa + d.chunk < d.nnx ? xa + d.chunk : d.nnx;
+    }
+}
+
+// Whether (col, row) lies outside the sponge frame, where P holds p.
+__host__ __device__ inline bool inner_cell(const ShotDev &d, int col, int row)
+{
+    return col >= d.xin_lo && col <= d.xin_hi && row >= d.zin_lo && row <= d.zin_hi;
 }
 
 // One shot of the adjoint (backward) time loop.  Tiles as in ShotDev.

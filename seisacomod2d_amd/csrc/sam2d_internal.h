@@ -51,7 +51,7 @@ struct DeviceDomain {
 
 // Device state of one resident shot slot (grown on demand, reused).
 struct ShotBuffers {
-    float *state = nullptr;        // nslots x cells x 4 fields (cell-interleaved)
+    float *state = nullptr;        // nslots x 4 x cells floats (slot_cells, slot_pz)
     float *amp = nullptr;
     int *gcell = nullptr;
     float *seis = nullptr;
@@ -150,6 +150,7 @@ void tile_shape(int nnz, int nnx, int waves, int chunk, int own_rows, int &ngrou
 // Indices [lo, hi] around the middle of an n-point axis where all four
 // sponge profiles of prof[4][n] are exactly 1 (lo > hi: none).
 void ones_range(const std::vector<float> &prof, int n, int &lo, int &hi);
+void shot_tiles(ShotDev &d, int variant, int chunk);
 // Descriptor of one shot for `variant` with march length `chunk`, tiles from
 // tile_begin; the device buffers are left for the caller to set.
 void describe_shot(const ShotPlan &p, int variant, int chunk, int tile_begin, int nts, int real_nt,

@@ -32,14 +32,14 @@ __device__ __forceinline__ void dplus_coef(int i, int n, float &c0, float &c1)
 // otherwise drain all loads in flight and serialise the column prefetch.
 #define SAM2D_GLOBAL __attribute__((address_space(1)))
 
-// The four state fields of a cell (px, pz, vx, vz) in the cell-interleaved
-// state layout: a native vector type, which global-address-space pointers
-// accept.
+// Four floats of a cell (the adjoint's coefficient cells with KZ): a native
+// vector type, which global-address-space pointers accept.
 typedef float fquad __attribute__((ext_vector_type(4)));
 template <class T>
 __device__ __forceinline__ SAM2D_GLOBAL T *gp(T *p) { return (SAM2D_GLOBAL T *)p; }
-// Three floats of a cell (the forward ring's triplets, the adjoint's
-// accumulators and coefficients), 4-byte aligned: one dwordx3 access per lane.
+// Three floats of a cell (the state's (P, vx, vz), the forward ring's
+// triplets, the adjoint's accumulators and coefficients), 4-byte aligned: one
+// dwordx3 access per lane.
 typedef float ftri __attribute__((ext_vector_type(3), aligned(4)));
 __device__ __forceinline__ ftri ld3(const SAM2D_GLOBAL char *__restrict__ base, unsigned off)
 {
@@ -128,18 +128,25 @@ __device__ __forceinline__ SAM2D_GLOBAL float *band_slice(const ShotDev &d, int 
                             (ptrdiff_t)d.nnx;
 }
 
+// Pressure of cell c = (col, row) in a state slot: P inside, P + pz in the
+// sponge frame (the sum the kernels formed in registers, bit for bit).
+__device__ __forceinline__ float state_pressure(const ShotDev &d, int slot, int c, int col, int row)
+{
+    const float P = gp(slot_cells(d, slot))[3 * (size_t)c];
+    return inner_cell(d, col, row) ? P : P + gp(slot_pz(d, slot))[c];
+}
+
 // Deferred receiver gather of the previous multi-step launch (cpu_record,
 // src/src_cpu/cpu_modelling_kernels.cpp:24-31, into the trace-major gather as
 // after cpu_transpose, :370-381): samples bs.x .. bs.w from band slices 0..3
-// of parity band_r, sample ss as px + pz of the state slot (each skipped when
-// < 0).
+// of parity band_r, sample ss as the pressure of the state slot (each skipped
+// when < 0).
 __device__ __forceinline__ void gather_deferred(const ShotDev *sh, int nshots, int band_r, int4 bs,
                                                 int state_slot, int ss, int gtid, int gstride)
 {
     const int smp[BAND_SLICES] = {bs.x, bs.y, bs.z, bs.w};
     for (int s = 0; s < nshots; ++s) {
         const ShotDev &d = sh[s];
-        const SAM2D_GLOBAL float *st = gp(slot_cells(d, state_slot));
         const SAM2D_GLOBAL int *gc = gp(d.gcell);
         SAM2D_GLOBAL float *seis = gp(d.seis);
         for (int g = gtid; g < d.ng; g += gstride) {
@@ -150,7 +157,7 @@ __device__ __forceinline__ void gather_deferred(const ShotDev *sh, int nshots, i
             #pragma unroll
             for (int j = 0; j < BAND_SLICES; j++)
                 if (smp[j] >= 0) tr[smp[j]] = band_slice(d, band_r, j)[at];
-            if (ss >= 0) tr[ss] = st[4 * (size_t)c] + st[4 * (size_t)c + 1];
+            if (ss >= 0) tr[ss] = state_pressure(d, state_slot, c, col, row);
         }
     }
 }

@@ -11,13 +11,14 @@
 //
 // Layout: cells are column-major with z fastest, cell (ix, iz) at
 // ix*nnz + iz (the reference's id = iz + ix*nnz), the state cell-interleaved
-// (px, pz, vx, vz of a cell together).  A wavefront owns a strip
+// ((P, vx, vz) of a cell together, P = px in the sponge frame and p = px + pz
+// outside it, pz a field of the frame; slot_cells).  A wavefront owns a strip
 // of OWN_Z rows and carries a HALO_T-row halo above and HALO_B rows below in
 // the same 64 lanes, so z-neighbours are lane shuffles.  It marches along x
 // over a chunk of columns keeping a sliding register window of vx, vz and the
 // new p; p is recomputed redundantly on the 3 halo rows and the 3 halo
-// columns it needs, so one pass reads px, pz, vx, vz, KX, BX, BZ once and
-// writes px, pz, vx, vz once: 44 bytes per cell update.  State is
+// columns it needs, so one pass reads the state, KX, BX, BZ once and writes
+// the state once: 36 bytes per cell update outside the frame.  State is
 // double-buffered (in -> out) because neighbouring tiles read the old state
 // of the cells a tile overwrites.
 
@@ -38,19 +39,20 @@ __device__ __forceinline__ void gather_receivers(const ShotDev *sh, int nshots, 
 {
     for (int s = 0; s < nshots; ++s) {
         const ShotDev &d = sh[s];
-        const SAM2D_GLOBAL float *st = gp(slot_cells(d, in_slot));
         const SAM2D_GLOBAL int *gc = gp(d.gcell);
         SAM2D_GLOBAL float *seis = gp(d.seis);
         for (int g = gtid; g < d.ng; g += gstride) {
-            const size_t c = 4 * (size_t)gc[g];
-            seis[(size_t)g * d.real_nt + slot] = st[c] + st[c + 1];
+            const int c = gc[g];
+            const int col = c / d.nnz;
+            seis[(size_t)g * d.real_nt + slot] = state_pressure(d, in_slot, c, col, c - col * d.nnz);
         }
     }
 }
 
 // Operands of one column that the march loads (one row per lane).
 struct ColOps {
-    fquad s;                    // state at column c+1, kept whole until consumed
+    ftri s;                     // cells (P, vx, vz) at column c+1, kept whole until consumed
+    float pz;                   // pz at column c+1 (frame lanes; 0 elsewhere)
     float kx, kz;               // at column c
     float dax, skx;             // x-profiles at column c (uniform)
     float bx, bz;               // BX, BZ at column j = c-2
@@ -58,13 +60,14 @@ struct ColOps {
 };
 
 template <bool NT>
-__device__ __forceinline__ void st4(SAM2D_GLOBAL fquad *p, unsigned i, float a, float b, float c,
-                                    float e)
+__device__ __forceinline__ void st3n(SAM2D_GLOBAL char *base, unsigned off, float a, float b,
+                                     float c)
 {
-    fquad v;
-    v.x = a; v.y = b; v.z = c; v.w = e;
-    if (NT) __builtin_nontemporal_store(v, p + i);
-    else p[i] = v;
+    ftri v;
+    v.x = a; v.y = b; v.z = c;
+    SAM2D_GLOBAL ftri *p = (SAM2D_GLOBAL ftri *)(base + off);
+    if (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
 }
 
 // One time step: state slot in_slot -> out_slot.  rec_slot >= 0 asks for the
@@ -86,9 +89,10 @@ __device__ __forceinline__ void st4(SAM2D_GLOBAL fquad *p, unsigned i, float a, 
 // finite in-grid value, and every such value meets a zero stencil coefficient
 // (the reference's edge rules) before it could reach an owned cell.
 //
-// One 16-byte load per column brings the state of column c+1, whose px, pz,
-// vz wait in windows; the v-phase at j stores all four fields of column j,
-// px and pz kept from its p-phase two columns earlier.
+// One 12-byte load per column brings the cells of column c+1 (a frame lane
+// also loads pz), whose P, pz, vz wait in windows; the v-phase at j stores
+// the cell of column j (and a frame lane's pz), P and pz kept from its
+// p-phase two columns earlier.
 template <int WAVES, int PF, bool NTS, bool RING>
 __global__ __launch_bounds__(WAVES * 64)
 void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_xcd,
@@ -113,15 +117,17 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         const int nnz = d.nnz, nnx = d.nnx;
         const int zb = (group * WAVES + wv) * OWN_Z;
         if (zb >= nnz) continue;                      // wave-uniform
-        const int xa = chunk * d.chunk;
-        const int xb = min(xa + d.chunk, nnx);
+        int xa, xb;
+        tile_columns(d, chunk, xa, xb);
 
         const int r = zb - HALO_T + lane;             // this lane's row
         const bool rown = (lane >= HALO_T) && (lane < HALO_T + OWN_Z) && (r < nnz);
         const unsigned rc = (unsigned)min(max(r, 0), nnz - 1);
 
-        const SAM2D_GLOBAL fquad *__restrict__ si = (const SAM2D_GLOBAL fquad *)gp(slot_cells(d, in_slot));
-        SAM2D_GLOBAL fquad *__restrict__ so = (SAM2D_GLOBAL fquad *)gp(slot_cells(d, out_slot));
+        const SAM2D_GLOBAL char *__restrict__ si = (const SAM2D_GLOBAL char *)gp(slot_cells(d, in_slot));
+        SAM2D_GLOBAL char *__restrict__ so = (SAM2D_GLOBAL char *)gp(slot_cells(d, out_slot));
+        const SAM2D_GLOBAL float *__restrict__ pzi = gp(slot_pz(d, in_slot));
+        SAM2D_GLOBAL float *__restrict__ pzo = gp(slot_pz(d, out_slot));
         SAM2D_GLOBAL char *__restrict__ ro =
             RING ? (SAM2D_GLOBAL char *)gp(d.ring + (size_t)ring_slot * 3 * d.ncell) : nullptr;
         const SAM2D_GLOBAL float *__restrict__ KX = gp(d.KX);
@@ -145,10 +151,24 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         auto colbase = [&](int col) -> unsigned {
             return (unsigned)min(max(col, 0), nnx - 1) * (unsigned)nnz;
         };
+        // Sponge frame: this lane's (clamped) row in the z-frame; a (clamped)
+        // column outside the x-frame.  A lane outside the frame reads pz at
+        // element 0 (one cache line for all of them) and takes 0.
+        const bool zfr = (int)rc < d.zin_lo || (int)rc > d.zin_hi;
+        auto xin = [&](int col) -> bool {
+            const int cc = min(max(col, 0), nnx - 1);
+            return cc >= d.xin_lo && cc <= d.xin_hi;
+        };
+        auto load_pz = [&](int col, unsigned base) -> float {
+            const bool fr = zfr || !xin(col);
+            const float v = pzi[fr ? base + rc : 0u];
+            return fr ? v : 0.0f;
+        };
         auto load_col = [&](int c) -> ColOps {
             ColOps o;
             const unsigned b1 = colbase(c + 1), b0 = colbase(c), bj = colbase(c - 2);
-            o.s = si[b1 + rc];
+            o.s = ld3(si + (size_t)12 * (b1 + rc), 0u);
+            o.pz = load_pz(c + 1, b1);
             o.kx = KX[b0 + rc]; o.kz = KZ[b0 + rc];
             o.bx = BX[bj + rc]; o.bz = BZ[bj + rc];
             const int cc = min(max(c, 0), nnx - 1), jc = min(max(c - 2, 0), nnx - 1);
@@ -165,8 +185,10 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         for (int k = 0; k < 4; k++) vxw[k] = pw[k] = pxw[k] = pzw[k] = vzw[k] = pxnw[k] = pznw[k] = 0.0f;
         #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const fquad sv = si[colbase(xa - 3 + k) + rc];
-            pxw[k] = sv.x; pzw[k] = sv.y; vxw[k] = sv.z; vzw[k] = sv.w;
+            const unsigned b = colbase(xa - 3 + k);
+            const ftri sv = ld3(si + (size_t)12 * (b + rc), 0u);
+            pxw[k] = sv.x; vxw[k] = sv.y; vzw[k] = sv.z;
+            pzw[k] = load_pz(xa - 3 + k, b);
         }
 
         ColOps q[PF];
@@ -183,9 +205,9 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
                 ColOps &cur = q[u % PF];
                 // ---- p-phase at column c (cpu_pml_p, :335-351; source, :135-140)
                 pxw[(u + 3) & 3] = cur.s.x;
-                pzw[(u + 3) & 3] = cur.s.y;
-                vxw[(u + 3) & 3] = cur.s.z;
-                vzw[(u + 3) & 3] = cur.s.w;
+                pzw[(u + 3) & 3] = cur.pz;
+                vxw[(u + 3) & 3] = cur.s.y;
+                vzw[(u + 3) & 3] = cur.s.z;
                 const float vx_m2 = vxw[u & 3], vx_m1 = vxw[(u + 1) & 3];
                 const float vx_0 = vxw[(u + 2) & 3], vx_p1 = vxw[(u + 3) & 3];
                 float xm0, xm1;
@@ -201,9 +223,11 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
                 const float pzn = fmaf(cur.kz * skz, dvz, daz * pz_old);
                 if (src_row && c == src_x) pxn += amp;
                 const float pn = pxn + pzn;
+                // outside the frame P carries p and pz stays 0
+                const bool inner = !zfr && xin(c);
                 pw[(u + 3) & 3] = pn;
-                pxnw[(u + 3) & 3] = pxn;              // stored with vx, vz at j
-                pznw[(u + 3) & 3] = pzn;
+                pxnw[(u + 3) & 3] = inner ? pn : pxn;  // stored with vx, vz at j
+                pznw[(u + 3) & 3] = inner ? 0.0f : pzn;
                 if (RING && rown && c >= xa && c < xb)
                     st3(ro, ((unsigned)c * (unsigned)nnz + (unsigned)r) * 12u, pn, vx_0, vz_0);
 
@@ -222,7 +246,8 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
                     const float vzn = fmaf(cur.bz * sbz, dpz, dbz * vz_j);
                     if (rown) {
                         const unsigned id = (unsigned)j * (unsigned)nnz + (unsigned)r;
-                        st4<NTS>(so, id, pxnw[(u + 1) & 3], pznw[(u + 1) & 3], vxn, vzn);
+                        st3n<NTS>(so + (size_t)12 * id, 0u, pxnw[(u + 1) & 3], vxn, vzn);
+                        if (zfr || !xin(j)) pzo[id] = pznw[(u + 1) & 3];
                     }
                 }
                 cur = load_col(min(c + PF, xb + 1));

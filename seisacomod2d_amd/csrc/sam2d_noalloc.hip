@@ -110,9 +110,10 @@ extern "C" int sam2d_forward_noalloc(sam2d_ctx *ctx, const sam2d_shot *shot, flo
                                            hipMemcpyHostToDevice, st));
         if (plan.ng) HIP_TRY(hipMemcpyAsync((void *)d.gcell, plan.gcell.data(),
                                             sizeof(int) * plan.ng, hipMemcpyHostToDevice, st));
-        // The array keeps the last call's contents: clear the state and the
-        // output (fill kernels, not memsets).
-        HIP_TRY(launch_fill(d.state, 4 * N, 0.0f, st));
+        // The array keeps the last call's contents: clear both state slots
+        // (the kernels read pz as 0 outside the sponge frame) and the output
+        // (fill kernels, not memsets).
+        HIP_TRY(launch_fill(d.state, 2 * 4 * N, 0.0f, st));
         if (plan.ng) HIP_TRY(launch_fill(seis_dev, (size_t)plan.ng * ta.real_nt, 0.0f, st));
         HIP_TRY(hipEventRecord(ctx->ev0, st));
         SAM2D_TRY(enqueue_steps_on(ctx, desc_dev, d.ngroups, 1, d.ngroups * d.nchunks, variant, 0,
