@@ -196,11 +196,14 @@ void tile_shape(int nnz, int nnx, int waves, int chunk, int own_rows, int &ngrou
 
 // Tiles of a shot for a variant and march length chunk (d.nnz, d.nnx and the
 // x-profile range xin_lo..xin_hi set): z-strip groups and chunk columns.  For
-// a multi-step variant the x-frame goes into chunk columns of its own,
+// a multi-step variant the x-frame goes into chunk columns of its own, over
 // [0, x0) and [x1, nnx), with x0 and x1 chosen so that the march of every
 // other tile lies in the kernel's fast range (fast_tile in
-// sam2d_kernelsn.h): those tiles carry no sponge-frame columns, and the
-// frame tiles, whose march takes the general path, are narrow.
+// sam2d_kernelsn.h): those tiles carry no sponge-frame columns.  The middle
+// part is cut into chunk-wide columns, the frame parts into half as wide ones:
+// the frame tiles take the general path, about 1.4 times the instructions per
+// column, and should finish no later than the others when all tiles of a
+// launch are resident at once (one small shot).
 void shot_tiles(ShotDev &d, int variant, int chunk)
 {
     int waves, vchunk, pf;
@@ -221,7 +224,10 @@ void shot_tiles(ShotDev &d, int variant, int chunk)
     if (w < 0) w = 0;
     d.x0 = x0;
     d.x1 = x0 + m * chunk + w;
-    d.nchunks = 2 + m + (w > 0 ? 1 : 0);
+    d.fchunk = std::max((chunk + 1) / 2, 8);
+    d.nlead = (x0 + d.fchunk - 1) / d.fchunk;
+    d.nmid = m + (w > 0 ? 1 : 0);
+    d.nchunks = d.nlead + d.nmid + (d.nnx - d.x1 + d.fchunk - 1) / d.fchunk;
 }
 
 int pick_variant(sam2d_ctx *ctx)

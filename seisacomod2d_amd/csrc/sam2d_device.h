@@ -25,9 +25,11 @@ struct ShotDev {
     int nnz, nnx;          // padded domain of this shot
     int ngroups, nchunks;  // tiles: z-strip groups x x-chunks
     int chunk;             // columns per x-chunk
-    int x0, x1;            // x0 < x1: chunk column 0 is [0, x0), the last [x1, nnx),
-                           //   those between are chunk-wide from x0 (the last of them
-                           //   ends at x1); otherwise chunk-wide from 0 (tile_columns)
+    int x0, x1;            // x0 < x1: the chunk columns split [0, x0), [x0, x1) and
+    int nlead, nmid;       //   [x1, nnx) into nlead columns fchunk wide, nmid chunk
+    int fchunk;            //   wide and the rest fchunk wide (the last of each part
+                           //   shorter); otherwise chunk-wide columns from 0
+                           //   (tile_columns)
     int tile_begin;        // first flat tile id of this shot
     int src_x, src_z;      // source cell (padded indices)
     int nts;               // wavelet length in steps
@@ -77,9 +79,11 @@ __host__ __device__ inline float *slot_pz(const ShotDev &d, int slot)
 __host__ __device__ inline void tile_columns(const ShotDev &d, int k, int &xa, int &xb)
 {
     if (d.x0 < d.x1) {
-        if (k == 0) { xa = 0; xb = d.x0; }
-        else if (k == d.nchunks - 1) { xa = d.x1; xb = d.nnx; }
-        else { xa = d.x0 + (k - 1) * d.chunk; xb = xa + d.chunk < d.x1 ? xa + d.chunk : d.x1; }
+        int lo = 0, hi = d.x0, w = d.fchunk;
+        if (k >= d.nlead + d.nmid) { k -= d.nlead + d.nmid; lo = d.x1; hi = d.nnx; }
+        else if (k >= d.nlead) { k -= d.nlead; lo = d.x0; hi = d.x1; w = d.chunk; }
+        xa = lo + k * w;
+        xb = xa + w < hi ? xa + w : hi;
     } else {
         xa = k * d.chunk;
         xb = xa + d.chunk < d.nnx ? xa + d.chunk : d.nnx;
