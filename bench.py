@@ -34,6 +34,7 @@ Separate workloads (each prints its own JSON line):
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -172,8 +173,12 @@ def measured_traffic(kernel, config):
                 found.append((cfg.strip(), float(n)))
             except ValueError:
                 pass
+    # A variant's name carries its nominal march length (_chunkNN), which the
+    # run's own "chunk" overrides: variants that differ only in it are the
+    # same kernel.
+    norm = lambda c: re.sub(r"_chunk\d+", "", c)
     for cfg, n in found:
-        if cfg == config:
+        if norm(cfg) == norm(config):
             return n, cfg
     return (found[0][1], found[0][0]) if found else (None, None)
 
@@ -261,7 +266,8 @@ def bench_plain(args, sam):
             "avg_launch_us": round(launch_ms * 1e3, 2),
             "effective_gbs": round(hbm_bytes * spl / (launch_ms * 1e-3) / 1e9, 1),
             "traffic_profiled_config": traffic_cfg,
-            "traffic_config_matches_run": traffic_cfg == this_cfg}
+            "traffic_config_matches_run": traffic_cfg is not None and
+                re.sub(r"_chunk\d+", "", traffic_cfg) == re.sub(r"_chunk\d+", "", this_cfg)}
     if traffic:
         dram = traffic / (launch_ms * 1e-3) / 1e9
         roof["dram_gbs"] = round(dram, 1)
