@@ -312,8 +312,8 @@ __device__ __forceinline__ void stepn_tile(const ShotDev &d, int lane, const Ste
                         const float b_m1 = ld(Bf, coff(jv - 1));
                         bu2 = last_row ? 0.0f : 0.5f * (lane_dn1(b_m1) + b_m1);
                     }
-                    bx = round_unbiased_prod(dtdx, bu2, cell_draw(RND_BX, jv, r));
-                    bz = round_unbiased_prod(dtdz, bu1, cell_draw(RND_BZ, jv, r));
+                    bx = round_unbiased_prod_pos(dtdx, bu2, cell_draw(RND_BX, jv, r));
+                    bz = round_unbiased_prod_pos(dtdz, bu1, cell_draw(RND_BZ, jv, r));
                     b_0 = b_1;
                     vx_old = vxin[i0];
                     vz_old = vzin[i0];

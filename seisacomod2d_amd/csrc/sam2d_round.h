@@ -70,15 +70,21 @@ __host__ __device__ inline float round_unbiased(double t, uint32_t h)
     return round_towards(n, (float)(t - (double)n), h);
 }
 
-// The exact product of two floats a * b (dt/dx and dt/dz are float
-// products, cpumodelling.cpp:180-181, and the staggered buoyancies are
-// floats): n = a * b rounded to nearest, rem = fma(a, b, -n) exactly.
-__host__ __device__ inline float round_unbiased_prod(float a, float b, uint32_t h)
+// The exact product of two floats a * b, known to be finite and not negative
+// (dt/dx and dt/dz are float products, cpumodelling.cpp:180-181, and the
+// staggered buoyancies are positive floats): n = a * b rounded to nearest,
+// rem = fma(a, b, -n) exactly, then as round_towards.  The guards of
+// round_towards can only fire with a zero remainder here, where both forms
+// return n (for n = 0 the neighbour is a NaN, which the compare rejects), so
+// the kernels that form the buoyancy coefficients themselves skip them.
+__host__ __device__ inline float round_unbiased_prod_pos(float a, float b, uint32_t h)
 {
 #pragma clang fp contract(off)
     const float n = a * b;
     const float rem = fmaf(a, b, -n);
-    return round_towards(n, rem, h);
+    const float other = f32_bits(u32_bits(n) + (rem > 0.0f ? 1u : 0xFFFFFFFFu));
+    const float spacing = fabsf(other - n);             // a power of two (NaN when n = 0)
+    return draw24(h) * spacing < fabsf(rem) ? other : n;
 }
 
 }  // namespace sam2d

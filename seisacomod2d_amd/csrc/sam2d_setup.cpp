@@ -144,8 +144,8 @@ void build_domain(const sam2d_grid &g, const TimeAxis &ta, const float *vp, cons
                 out.KZ[i] = round_unbiased(ta.dtdz * out.kappa[i], cell_draw(RND_KZ, ix, iz));
             // dtdx, dtdz are float products widened to double: dtdx * bu2 is
             // the exact product of two floats
-            out.BX[i] = round_unbiased_prod((float)ta.dtdx, bu2[i], cell_draw(RND_BX, ix, iz));
-            out.BZ[i] = round_unbiased_prod((float)ta.dtdz, bu1[i], cell_draw(RND_BZ, ix, iz));
+            out.BX[i] = round_unbiased_prod_pos((float)ta.dtdx, bu2[i], cell_draw(RND_BX, ix, iz));
+            out.BZ[i] = round_unbiased_prod_pos((float)ta.dtdz, bu1[i], cell_draw(RND_BZ, ix, iz));
         }
 
     out.prof_x.assign(4 * (size_t)nnx, 0.0f);
