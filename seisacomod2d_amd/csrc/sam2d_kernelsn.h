@@ -268,7 +268,10 @@ __device__ __forceinline__ void stepn_tile(const ShotDev &d, int lane, const Ste
                     pxn = fmaf(kx * prx[nnx + cc], dvx, prx[cc] * px_old);
                 }
                 if (src_row && cp == srcx[t]) pxn += amp[t];
-                const float pzn = fmaf(kz * skz, dvz, daz * pz_old);
+                // MODE 2: every z-profile factor of the wave is exactly 1 and
+                // pz is 0, so the general form reduces to these (same bits)
+                const float pzn = MODE == 2 ? fmaf(kz, dvz, 0.0f)
+                                            : fmaf(kz * skz, dvz, daz * pz_old);
                 const float pn = pxn + pzn;
                 // outside the frame P carries p and pz stays 0
                 const bool inner = MODE == 2 ? true : (MODE == 1 ? !zfr : !zfr && xin(cp));
@@ -330,7 +333,8 @@ __device__ __forceinline__ void stepn_tile(const ShotDev &d, int lane, const Ste
                         const int jc = clampc(jv);
                         VX[t][i3] = fmaf(bx * prx[3 * nnx + jc], dpx, prx[2 * nnx + jc] * vx_old);
                     }
-                    VZ[t][i3] = fmaf(bz * sbz, dpz, dbz * vz_old);
+                    VZ[t][i3] = MODE == 2 ? fmaf(bz, dpz, vz_old)
+                                          : fmaf(bz * sbz, dpz, dbz * vz_old);
                     BXw[t][i3] = bx;
                     BZw[t][i3] = bz;
                 } else if (rown && jv >= xa && jv < xb) {
@@ -341,7 +345,8 @@ __device__ __forceinline__ void stepn_tile(const ShotDev &d, int lane, const Ste
                         const int jc = clampc(jv);
                         vxn = fmaf(bx * prx[3 * nnx + jc], dpx, prx[2 * nnx + jc] * vx_old);
                     }
-                    const float vzn = fmaf(bz * sbz, dpz, dbz * vz_old);
+                    const float vzn = MODE == 2 ? fmaf(bz, dpz, vz_old)
+                                                : fmaf(bz * sbz, dpz, dbz * vz_old);
                     const unsigned cn = (unsigned)jv * (unsigned)nnz;
                     st3(so, cn * 12u + rc12, PX[t][i1], vxn, vzn);
                     if (MODE != 2) st1(so, pzb + cn * 4u + rc4, PZ[t][i1]);
