@@ -6,8 +6,9 @@
 //   2. backward, segment by segment from the last: restore checkpoint m into
 //      state slot 0, recompute steps [m*k, m*k + k) (bit-identical to the
 //      first pass: the same arithmetic per cell, three steps per launch where
-//      a three-step variant fits), storing for each step n the triplet
-//      (p after n, vx and vz before n) of every cell into ring slot n - m*k,
+//      a three-step variant fits), storing for each step n the pressure
+//      after it and the dJ/dkappa factors of the state before it (ring_pair,
+//      ring_side in sam2d_device.h) into ring slot n - m*k,
 //      then run the adjoint steps of the segment in reverse, two per pass
 //      where every shot has a residual band, step n reading ring slot
 //      n - m*k;
@@ -28,7 +29,7 @@ struct AdjShotBuffers {
     float *lam = nullptr;     // 2 adjoint slots of (L_px, L_vx, L_vz) cells, 2 of L_pz,
                               // then the accumulator cells (dJ/dkappa, dJ/dbu2, dJ/dbu1)
     float *ckpt = nullptr;    // nckpt state slots (cell-interleaved)
-    float *ring = nullptr;    // k triplet slots (ShotDev::ring)
+    float *ring = nullptr;    // k ring slots of 12 bytes per cell (ShotDev::ring)
     float *srcg = nullptr;
     int *rcell = nullptr, *rptr = nullptr, *rid = nullptr;
     float *res = nullptr;
@@ -132,12 +133,13 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     const int ns = (int)plans.size();
     // Tile shape of the adjoint and of the single-step recomputation: the
     // "adjoint_variant" option, else the fastest shape measured for the C4
-    // gradient on MI355X (two steps: 2 waves x 256 columns, prefetch 2, 1398
-    // us per pass against 1459-1599 for the prefetch-4 shapes, which need
-    // more than 256 registers, and 1465 for 4 waves x 32 columns; one step:
-    // 1 wave x 64 columns, prefetch 2).
+    // gradient (32 shots, k = 100) on MI355X: two steps, 1 wave x 128
+    // columns, prefetch 2 (9.78-9.83 s per gradient with prefetch 2 or 4,
+    // 9.84-9.88 for 2 waves x 64-128 columns, 10.34 for 2 waves x 256, the
+    // default while the forward ring held three floats per cell); one step:
+    // 1 wave x 64 columns, prefetch 2.
     const int variant = ctx->adjoint_variant >= 0 ? ctx->adjoint_variant
-                        : ctx->adjoint_steps == 2 ? variant_index(2, 256, 2)
+                        : ctx->adjoint_steps == 2 ? variant_index(1, 128, 2)
                                                   : variant_index(1, 64, 2);
     int ntiles = 0;
     SAM2D_TRY(stage_batch(ctx, plans, variant, &ntiles, 2));

@@ -16,11 +16,15 @@
 //   out:  L_px = dax L_px',  L_pz = daz L_pz'                     (p sponge)
 //         L_vx = dbx L_vx + Dx-^T (KXe L_px')                     (v sponge, p-update)
 //         L_vz = dbz L_vz + Dz-^T (KZe L_pz')
-//   dJ/dkappa += L_px' (Dx- vx[n-1]) dtdx skx + L_pz' (Dz- vz[n-1]) dtdz skz
-//                (+ L_px'[src] fd_dt w[n]/(dx dz) at the source)
+//   dJ/dkappa += L_px' Ex + L_pz' Ez, Ex = (Dx- vx[n-1]) dtdx skx,
+//                Ez = (Dz- vz[n-1]) dtdz skz (+ L_px'[src] fd_dt w[n]/(dx dz)
+//                at the source)
 //   dJ/dbu2   += L_vx (Dx+ p[n]) dtdx sbx,   dJ/dbu1 += L_vz (Dz+ p[n]) dtdz sbz
 //
 // where KXe = KX*skx etc. are the forward's effective coefficients.  The
+// forward's recomputation stores p[n] and the factors of dJ/dkappa per cell
+// (ring_pair, ring_side): outside the sponge frame L_px' = L_pz' and one
+// factor E = Ex + Ez suffices; in the frame E = Ex and Ez apart.  The
 // transposes are written as gathers: Dx+^T g at column c is
 // c0(c-1) g[c-1] - c0(c) g[c] + c1(c-2) g[c-2] - c1(c+1) g[c+1] with the
 // forward's per-index coefficients (zero outside the grid), so the edge rules
@@ -147,9 +151,9 @@ hipError_t launch_pack_coef(const float *kx, const float *kz, const float *bx, c
 // arrives as one 12-byte load per lane at column c+1 (kept whole in the
 // prefetch slot) and leaves as one 12-byte store at the phase-B column, L_px
 // kept from phase A; frame lanes add a 4-byte load and store.  The forward
-// triplet of step n (p[n], vx[n-1], vz[n-1]) and the coefficient cell (KX,
-// BX, BZ[, KZ]) arrive at column c+1 as well, their column-c parts waiting a
-// column in windows; the accumulator cell (dJ/dkappa, dJ/dbu2, dJ/dbu1)
+// ring pair of step n (p[n], E) and the coefficient cell (KX, BX, BZ[, KZ])
+// arrive at column c+1 as well (a frame lane adds Ez), their column-c parts
+// waiting a column in windows; the accumulator cell (dJ/dkappa, dJ/dbu2, dJ/dbu1)
 // arrives at column c and leaves with the adjoint state at the phase-B
 // column.
 template <bool KZA>
@@ -159,7 +163,8 @@ template <bool KZA>
 struct AdjCol {
     ftri lam;                   // column c+1: adjoint L_px, L_vx, L_vz
     float lpz;                  //             L_pz (sponge frame)
-    ftri fw;                    //             p[n], vx[n-1], vz[n-1]
+    fpair fw;                   //             p[n], E (ring_pair)
+    float ez;                   //             Ez (sponge frame, ring_side)
     CoefCell<KZA> cf;           //             KX, BX, BZ[, KZ]
     ftri acc;                   // column c:   dJ/dkappa, dJ/dbu2, dJ/dbu1
     float res;                  //             residual of this step
@@ -207,6 +212,7 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         gcchar *__restrict__ Zin = (gcchar *)(d.lpz + (size_t)lam_in * N);
         gchar *__restrict__ Zout = (gchar *)(d.lpz + (size_t)lam_out * N);
         gcchar *__restrict__ FW = (gcchar *)(d.ring + 3 * (size_t)fslot * N);
+        gcchar *__restrict__ FWZ = (gcchar *)(d.ring + 3 * (size_t)fslot * N + 2 * N);
         gcchar *__restrict__ CF = (gcchar *)d.coef;
         gchar *__restrict__ ACC = (gchar *)d.acc;
         const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);
@@ -267,7 +273,9 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             o.lam = ld3(Lin, 3 * o1);
             o.lpz = 0.0f;
             if (frame(c + 1)) o.lpz = ldf(Zin, o1);
-            o.fw = ld3(FW, 3 * o1);
+            o.fw = ld2(FW, 2 * o1);
+            o.ez = 0.0f;
+            if (frame(c + 1)) o.ez = ldf(FWZ, o1);
             o.cf = ldcf(o1);
             o.acc = ld3(ACC, 3 * o0);
             o.res = -0.0f;
@@ -287,16 +295,16 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
         };
 
         // vx-style windows: [k & 3] = column c-2+k (LVX, LPX, LPZ, LVZ: the
-        // input adjoint state; FVX, FVZ, FP: the forward triplet; KXn, KZn,
+        // input adjoint state; FP, FE, FEZ: the forward ring; KXn, KZn,
         // BZn: coefficients; GBX, GBZ: accumulators as loaded; GK: dJ/dkappa
         // after phase A).  p-style: [k & 3] = column c-3+k (OPX, OPZ: the new
         // L_px, L_pz, stored with phase B).
-        float LVX[4], LPX[4], LPZ[4], LVZ[4], FVX[4], FVZ[4], FP[4], AXP[4], BXP[4];
+        float LVX[4], LPX[4], LPZ[4], LVZ[4], FE[4], FEZ[4], FP[4], AXP[4], BXP[4];
         float KXn[4], KZn[4], BZn[4], GBX[4], GBZ[4], GK[4];
         float AM[4], BM[4], AMZ[4], BMZ[4], PN[4], OPX[4], OPZ[4];
         #pragma unroll
         for (int k = 0; k < 4; k++) {
-            LVX[k] = LPX[k] = LPZ[k] = LVZ[k] = FVX[k] = FVZ[k] = FP[k] = AXP[k] = BXP[k] = 0.0f;
+            LVX[k] = LPX[k] = LPZ[k] = LVZ[k] = FE[k] = FEZ[k] = FP[k] = AXP[k] = BXP[k] = 0.0f;
             KXn[k] = KZn[k] = BZn[k] = GBX[k] = GBZ[k] = GK[k] = 0.0f;
             AM[k] = BM[k] = AMZ[k] = BMZ[k] = PN[k] = OPX[k] = OPZ[k] = 0.0f;
         }
@@ -307,8 +315,9 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             const ftri lv = ld3(Lin, 3 * o);
             LPX[k] = lv.x; LVX[k] = lv.y; LVZ[k] = lv.z;
             LPZ[k] = frame(col) ? ldf(Zin, o) : lv.x;
-            const ftri fw = ld3(FW, 3 * o);
-            FP[k] = fw.x; FVX[k] = fw.y; FVZ[k] = fw.z;
+            const fpair fw = ld2(FW, 2 * o);
+            FP[k] = fw.x; FE[k] = fw.y;
+            FEZ[k] = frame(col) ? ldf(FWZ, o) : 0.0f;
             const CoefCell<KZA> cf = ldcf(o);
             KXn[k] = cf.x; BZn[k] = cf.z;
             if constexpr (!KZA) KZn[k] = cf.w;
@@ -333,8 +342,8 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             const float lvx_n = keep(cur.lam.y);
             LVX[i3] = lvx_n;
             FP[i3] = keep(cur.fw.x);
-            FVX[i3] = keep(cur.fw.y);
-            FVZ[i3] = keep(cur.fw.z);
+            FE[i3] = keep(cur.fw.y);
+            FEZ[i3] = frame(c + 1) ? keep(cur.ez) : 0.0f;
             const float bx_n = keep(cur.cf.y);
             KXn[i3] = keep(cur.cf.x);
             BZn[i3] = keep(cur.cf.z);
@@ -358,10 +367,6 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
             const float lpz1 = (LPZ[i2] + cur.res) + lp;
             float xm0 = SAM2D_A0, xm1 = SAM2D_A1;
             if (!FAST) dminus_coef(c, nnx, xm0, xm1);
-            // dJ/dkappa from the p-update needs Dx- vx[n-1] and Dz- vz[n-1];
-            // the lane shifts run with every lane active.
-            const float f0 = FVZ[i2];
-            const float f_u1 = lane_up1(f0), f_u2 = lane_up1(f_u1), f_d1 = lane_dn1(f0);
             float skx = 1.0f, dax = 1.0f;
             if (!FAST) {
                 const int cc = clampc(c);
@@ -369,9 +374,9 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
                 skx = prx[nnx + cc];
             }
             {
-                const float dvx = fmaf(xm0, FVX[i2] - FVX[i1], xm1 * (FVX[i3] - FVX[i0]));
-                const float dvz = fmaf(zm0, f0 - f_u1, zm1 * (f_d1 - f_u2));
-                float g = cur.acc.x + lpx1 * dvx * (dtdx * skx) + lpz1 * dvz * (dtdz * skz);
+                // dJ/dkappa of the p-update, from the forward's factors
+                const float gk = cur.acc.x + lpx1 * FE[i2];
+                float g = frame(c) ? gk + lpz1 * FEZ[i2] : gk;
                 if (src_lane && c == src_x) g += lpx1 * srcg;
                 GK[i2] = g;
             }
@@ -455,18 +460,19 @@ void k_adjoint(const AdjDev *__restrict__ shots, int nshots, int vtotal, int per
 // with the same arithmetic per cell as k_adjoint, so both give the same bits.
 // The x-transposes are formed at use from windows of the per-column factors
 // (G = BXe L_vx, H = KXe L_px', HZ = KZe L_pz') and per-index coefficients.
-// Forward triplets: step n's (p[n], vx[n-1], vz[n-1]) in ring slot fslot,
-// loaded at column c+1; step n-1's (p[n-1], vx[n-2], vz[n-2]) in fslot - 1,
-// loaded at column c-2.
+// Forward ring: step n's pairs (p[n], E) in ring slot fslot, loaded at column
+// c+1; step n-1's in fslot - 1, loaded at column c-2 (frame lanes add Ez).
 template <bool KZA>
 struct AdjCol2 {
     ftri lam;                   // column c+1: adjoint L_px, L_vx, L_vz
     float lpz;                  //             L_pz (sponge frame)
-    ftri fw1;                   //             forward triplet of step n
+    fpair fw1;                  //             forward ring pair of step n
+    float ez1;                  //             its Ez (sponge frame)
     CoefCell<KZA> cf;           //             KX, BX, BZ[, KZ]
     ftri acc;                   // column c:   dJ/dkappa, dJ/dbu2, dJ/dbu1
     float res1;                 //             residual of step n
-    ftri fw2;                   // column c-2: forward triplet of step n-1
+    fpair fw2;                  // column c-2: forward ring pair of step n-1
+    float ez2;                  //             its Ez (sponge frame)
     float res2;                 // column c-3: residual of step n-1
 };
 
@@ -510,6 +516,8 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         gchar *__restrict__ Zout = (gchar *)(d.lpz + (size_t)lam_out * N);
         gcchar *__restrict__ FW1 = (gcchar *)(d.ring + 3 * (size_t)fslot * N);
         gcchar *__restrict__ FW2 = (gcchar *)(d.ring + 3 * (size_t)(fslot - 1) * N);
+        gcchar *__restrict__ FWZ1 = FW1 + 8 * N;      // ring_side of each slot
+        gcchar *__restrict__ FWZ2 = FW2 + 8 * N;
         gcchar *__restrict__ CF = (gcchar *)d.coef;
         gchar *__restrict__ ACC = (gchar *)d.acc;
         const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);
@@ -577,10 +585,14 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
             o.lam = ld3(Lin, 3 * o1);
             o.lpz = 0.0f;
             if (frame(c + 1)) o.lpz = ldf(Zin, o1);
-            o.fw1 = ld3(FW1, 3 * o1);
+            o.fw1 = ld2(FW1, 2 * o1);
+            o.ez1 = 0.0f;
+            if (frame(c + 1)) o.ez1 = ldf(FWZ1, o1);
             o.cf = ldcf(o1);
             o.acc = ld3(ACC, 3 * o0);
-            o.fw2 = ld3(FW2, 3 * oj);
+            o.fw2 = ld2(FW2, 2 * oj);
+            o.ez2 = 0.0f;
+            if (frame(c - 2)) o.ez2 = ldf(FWZ2, oj);
             o.res1 = -0.0f;
             o.res2 = -0.0f;
             if (band1) {
@@ -593,28 +605,28 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
         };
 
         // Windows, [k & 3] relative to the unrolled base (columns held):
-        //   G1 LVX0 FVX1 BXW                      c-2 .. c+1
-        //   FP1n FVZ1n KXn KZn BZn (as loaded)    c-2 .. c+1
+        //   G1 LVX0 BXW                           c-2 .. c+1
+        //   FP1n FE1 FEZ1 KXn KZn BZn (as loaded) c-2 .. c+1
         //   LVZ0 BZW KXW KZW LPX1 LPZ1 GKW H1 HZ1 PN1   c-3 .. c
         //   GBXn GBZn (accumulators as loaded)    c-3 .. c
-        //   G2 LVX1 LVZ1 GBXW GBZW FVX2           c-5 .. c-2
-        //   FP2n FVZ2n (as loaded)                c-5 .. c-2
+        //   G2 LVX1 LVZ1 GBXW GBZW                c-5 .. c-2
+        //   FP2n FE2 FEZ2 (as loaded)             c-5 .. c-2
         //   H2 HZ2 PN2 GK2                        c-6 .. c-3
         //   LPXn LPZn LVZn (input L_px, L_pz, L_vz)  c-2 .. c+1
         //   OPA OPZA (new L_px, L_pz of step n-1)   c-6 .. c-3, stored with B2
-        float G1[4], LVX0[4], FVX1[4], BXW[4], LPXn[4], LPZn[4], LVZn[4];
-        float FP1n[4], FVZ1n[4], KXn[4], KZn[4], BZn[4], GBXn[4], GBZn[4], FP2n[4], FVZ2n[4];
+        float G1[4], LVX0[4], FE1[4], BXW[4], LPXn[4], LPZn[4], LVZn[4];
+        float FP1n[4], FEZ1[4], KXn[4], KZn[4], BZn[4], GBXn[4], GBZn[4], FP2n[4], FEZ2[4];
         float LVZ0[4], BZW[4], KXW[4], KZW[4], LPX1[4], LPZ1[4], GKW[4], H1[4], HZ1[4], PN1[4];
-        float G2[4], LVX1[4], LVZ1[4], GBXW[4], GBZW[4], FVX2[4];
+        float G2[4], LVX1[4], LVZ1[4], GBXW[4], GBZW[4], FE2[4];
         float H2[4], HZ2[4], PN2[4], GK2[4], OPA[4], OPZA[4];
         #pragma unroll
         for (int k = 0; k < 4; k++) {
-            G1[k] = LVX0[k] = FVX1[k] = BXW[k] = LPXn[k] = LPZn[k] = LVZn[k] = 0.0f;
-            FP1n[k] = FVZ1n[k] = KXn[k] = KZn[k] = BZn[k] = GBXn[k] = GBZn[k] = 0.0f;
-            FP2n[k] = FVZ2n[k] = 0.0f;
+            G1[k] = LVX0[k] = FE1[k] = BXW[k] = LPXn[k] = LPZn[k] = LVZn[k] = 0.0f;
+            FP1n[k] = FEZ1[k] = KXn[k] = KZn[k] = BZn[k] = GBXn[k] = GBZn[k] = 0.0f;
+            FP2n[k] = FEZ2[k] = 0.0f;
             LVZ0[k] = BZW[k] = KXW[k] = KZW[k] = LPX1[k] = LPZ1[k] = GKW[k] = 0.0f;
             H1[k] = HZ1[k] = PN1[k] = 0.0f;
-            G2[k] = LVX1[k] = LVZ1[k] = GBXW[k] = GBZW[k] = FVX2[k] = 0.0f;
+            G2[k] = LVX1[k] = LVZ1[k] = GBXW[k] = GBZW[k] = FE2[k] = 0.0f;
             H2[k] = HZ2[k] = PN2[k] = GK2[k] = OPA[k] = OPZA[k] = 0.0f;
         }
         const int c_first = xa - 4;
@@ -625,8 +637,9 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
             const ftri lv = ld3(Lin, 3 * o);
             LPXn[k] = lv.x; LVX0[k] = lv.y; LVZn[k] = lv.z;
             LPZn[k] = frame(col) ? ldf(Zin, o) : lv.x;
-            const ftri fw = ld3(FW1, 3 * o);
-            FP1n[k] = fw.x; FVX1[k] = fw.y; FVZ1n[k] = fw.z;
+            const fpair fw = ld2(FW1, 2 * o);
+            FP1n[k] = fw.x; FE1[k] = fw.y;
+            FEZ1[k] = frame(col) ? ldf(FWZ1, o) : 0.0f;
             const CoefCell<KZA> cf = ldcf(o);
             KXn[k] = cf.x; BXW[k] = cf.y; BZn[k] = cf.z;
             if constexpr (!KZA) KZn[k] = cf.w;
@@ -652,8 +665,8 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                 const float lvx_n = keep(cur.lam.y);
                 LVX0[i3] = lvx_n;
                 FP1n[i3] = keep(cur.fw1.x);
-                FVX1[i3] = keep(cur.fw1.y);
-                FVZ1n[i3] = keep(cur.fw1.z);
+                FE1[i3] = keep(cur.fw1.y);
+                FEZ1[i3] = frame(c + 1) ? keep(cur.ez1) : 0.0f;
                 const float bxn = keep(cur.cf.y);
                 BXW[i3] = bxn;
                 KXn[i3] = keep(cur.cf.x);
@@ -688,11 +701,8 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                     dax = prx[cc];
                     skx = prx[nnx + cc];
                 }
-                const float f0 = FVZ1n[i2];
-                const float f_u1 = lane_up1(f0), f_u2 = lane_up1(f_u1), f_d1 = lane_dn1(f0);
-                const float dvx = fmaf(xm0, FVX1[i2] - FVX1[i1], xm1 * (FVX1[i3] - FVX1[i0]));
-                const float dvz = fmaf(zm0, f0 - f_u1, zm1 * (f_d1 - f_u2));
-                float g = cur.acc.x + lpx1 * dvx * (dtdx * skx) + lpz1 * dvz * (dtdz * skz);
+                const float gk = cur.acc.x + lpx1 * FE1[i2];
+                float g = frame(c) ? gk + lpz1 * FEZ1[i2] : gk;
                 if (src_lane && c == src_x1) g += lpx1 * srcg1;
                 GKW[i3] = g;
                 LPX1[i3] = dax * lpx1;
@@ -739,8 +749,8 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                 GBZW[i3] = GBZn[i1] + lvz_j * dpz * (dtdz * sbz);
                 G2[i3] = (BXW[i0] * sbx) * ovx;
                 FP2n[i3] = keep(cur.fw2.x);
-                FVX2[i3] = keep(cur.fw2.y);
-                FVZ2n[i3] = keep(cur.fw2.z);
+                FE2[i3] = keep(cur.fw2.y);
+                FEZ2[i3] = frame(c - 2) ? keep(cur.ez2) : 0.0f;
             }
 
             // ---- A2: step n-1, phase A at column c2 = c-3 (results stored)
@@ -768,12 +778,9 @@ void k_adjoint2(const AdjDev *__restrict__ shots, int nshots, int vtotal, int pe
                     dax = prx[cc];
                     skx = prx[nnx + cc];
                 }
-                const float f0 = FVZ2n[i2];
-                const float f_u1 = lane_up1(f0), f_u2 = lane_up1(f_u1), f_d1 = lane_dn1(f0);
                 {
-                    const float dvx = fmaf(xm0, FVX2[i2] - FVX2[i1], xm1 * (FVX2[i3] - FVX2[i0]));
-                    const float dvz = fmaf(zm0, f0 - f_u1, zm1 * (f_d1 - f_u2));
-                    float g = GKW[i0] + lpx2 * dvx * (dtdx * skx) + lpz2 * dvz * (dtdz * skz);
+                    const float gk = GKW[i0] + lpx2 * FE2[i2];
+                    float g = frame(c2) ? gk + lpz2 * FEZ2[i2] : gk;
                     if (src_lane && c2 == src_x2) g += lpx2 * srcg2;
                     GK2[i3] = g;                  // stored with phase B2
                 }

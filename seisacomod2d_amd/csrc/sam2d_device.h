@@ -47,9 +47,8 @@ struct ShotDev {
     const float *prof_z;   // [4][nnz]: damp1a, damp1a1, damp1b, damp1b1
     float *state;          // state slots of 4*ncell floats (slot_cells, slot_pz)
     size_t ncell;
-    float *ring;           // recomputation only: forward ring of triplets, slot m
-                           //   (segment step m) at ring + 3*m*ncell, cell i holding
-                           //   p after step m and vx, vz before it at [3i .. 3i+2]
+    float *ring;           // recomputation only: the forward ring, slot m (segment
+                           //   step m) at ring + 3*m*ncell (ring_pair, ring_side)
     const int *gcell;      // [ng] receiver cells (ix*nnz + iz)
     float *seis;           // [ng][real_nt]
     float *band;           // [2][BAND_SLICES][band_h][nnx]: pressure of the receiver
@@ -90,6 +89,23 @@ __host__ __device__ inline void tile_columns(const ShotDev &d, int k, int &xa, i
     }
 }
 
+// The forward ring: slot m holds, for segment step m, what the backward step
+// of the same index reads of the forward: for cell i the pair (p, E) at
+// [2i, 2i+1] (ring_pair) and, in the sponge frame only, Ez at [2N + i]
+// (ring_side).  p is the pressure after the step; with Ex = Dx- vx (dt/dx skx)
+// and Ez = Dz- vz (dt/dz skz) of the state before it (the p-sponge scale
+// factors skx, skz), E is Ex + Ez outside the frame and Ex in it.  These are
+// the dJ/dkappa factors of the step: outside the frame L_px = L_pz there and
+// dJ/dkappa takes L_px E; in the frame, L_px Ex + L_pz Ez.
+__host__ __device__ inline float *ring_pair(float *ring, size_t ncell, int slot)
+{
+    return ring + (size_t)slot * 3 * ncell;
+}
+__host__ __device__ inline float *ring_side(float *ring, size_t ncell, int slot)
+{
+    return ring + (size_t)slot * 3 * ncell + 2 * ncell;
+}
+
 // Whether (col, row) lies outside the sponge frame, where P holds p.
 __host__ __device__ inline bool inner_cell(const ShotDev &d, int col, int row)
 {
@@ -109,7 +125,7 @@ struct AdjDev {
     const float *coef;     // KX, BX, BZ (and KZ unless it equals KX) of cell i at
                            //   [3i .. 3i+2] ([4i .. 4i+3] with KZ last)
     const float *prof_x, *prof_z;
-    const float *ring;     // forward ring of triplets (ShotDev::ring)
+    const float *ring;     // forward ring (ShotDev::ring: ring_pair, ring_side)
     float *lam;            // adjoint state: 2 slots of 3*ncell floats, L_px, L_vx, L_vz
                            //   of cell i at [3i .. 3i+2]
     float *lpz;            // L_pz: 2 slots of ncell floats, read and written only in the
@@ -153,14 +169,14 @@ inline int per_xcd_for(int ntiles, TileOrder &order)
     return (ntiles + 7) / 8;
 }
 // One backward step n of every staged shot: adjoint state lam_in -> lam_out,
-// forward triplets of step n in ring slot fslot; rec_k >= 0 injects residual
+// the forward ring slot of step n is fslot; rec_k >= 0 injects residual
 // sample rec_k of the shots that carry an rband.  kz_alias: every staged
 // shot's KZ is its KX (three-float coefficient cells).
 hipError_t launch_adjoint_step(int variant, const AdjDev *d_adj, int nshots, int ntiles,
                                bool kz_alias, int step, int lam_in, int lam_out, int fslot,
                                int rec_k, TileOrder order, hipStream_t stream);
 // Backward steps n and n-1 in one pass (all shots must carry an rband or no
-// receivers): forward triplets of step n in ring slot fslot, of step n-1 in
+// receivers): the forward ring slot of step n is fslot, of step n-1
 // fslot - 1; residual samples rec1 (step n) and rec2 (step n-1).  Tiles of
 // OWN_Z2-row strips.
 hipError_t launch_adjoint_step2(int variant, const AdjDev *d_adj, int nshots, int ntiles,
@@ -214,9 +230,9 @@ hipError_t launch_stepn(int variant, const ShotDev *d_shots, int nshots, int nti
                         int band_w, const int *band_samples, int state_sample, int max_ng,
                         TileOrder order, hipStream_t stream);
 // The recomputation form of a three-step variant: state slot in_slot ->
-// in_slot ^ 1, and for each of the three steps step+t the triplet (p after
-// it, vx and vz before it) of every owned cell into ring slot ring_slot + t
-// (no receivers).
+// in_slot ^ 1, and for each of the three steps step+t the ring entries of
+// every owned cell (p after it and the dJ/dkappa factors of the state before
+// it, ring_pair / ring_side) into ring slot ring_slot + t (no receivers).
 bool ring_supported(int variant);
 hipError_t launch_stepn_ring(int variant, const ShotDev *d_shots, int nshots, int ntiles,
                              int grid_cap, bool kz_alias, int step, int in_slot, int ring_slot,
@@ -232,7 +248,7 @@ hipError_t launch_gather(const ShotDev *d_shots, int nshots, int max_ng, int ban
 hipError_t launch_step(int variant, const ShotDev *d_shots, int nshots, int ntiles, int grid_cap,
                        int step, int in_slot, int out_slot, int rec_sample, hipStream_t stream);
 // The recomputation form of one step: in_slot -> out_slot and the step's
-// triplets into ring slot ring_slot; the single-step variant's tile shape
+// ring entries into ring slot ring_slot; the single-step variant's tile shape
 // (stores without the non-temporal hint), no receivers.
 hipError_t launch_step_ring(int variant, const ShotDev *d_shots, int nshots, int ntiles,
                             int grid_cap, int step, int in_slot, int out_slot, int ring_slot,

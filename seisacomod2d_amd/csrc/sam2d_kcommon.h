@@ -52,6 +52,32 @@ __device__ __forceinline__ void st3(SAM2D_GLOBAL char *__restrict__ base, unsign
     v.x = a; v.y = b; v.z = c;
     *(SAM2D_GLOBAL ftri *__restrict__)(base + off) = v;
 }
+// Two floats of a cell (the forward ring's (p, E) pairs), 8-byte aligned: one
+// dwordx2 access per lane.
+typedef float fpair __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ fpair ld2(const SAM2D_GLOBAL char *__restrict__ base, unsigned off)
+{
+    return *(const SAM2D_GLOBAL fpair *__restrict__)(base + off);
+}
+__device__ __forceinline__ void st2(SAM2D_GLOBAL char *__restrict__ base, unsigned off, float a,
+                                    float b)
+{
+    fpair v;
+    v.x = a; v.y = b;
+    *(SAM2D_GLOBAL fpair *__restrict__)(base + off) = v;
+}
+// The forward ring's dJ/dkappa factors of one step at one cell (ring_pair,
+// ring_side): Ex = dvx (dtdx skx), Ez = dvz (dtdz skz); returns E, Ex + Ez
+// outside the sponge frame, Ex in it.  Every product and sum rounded as
+// written, so the single- and multi-step recomputations store the same bits.
+__device__ __forceinline__ float ring_factor(float dvx, float dtdx_skx, float dvz, float dtdz_skz,
+                                             bool inner, float &ez)
+{
+#pragma clang fp contract(off)
+    const float ex = dvx * dtdx_skx;
+    ez = dvz * dtdz_skz;
+    return inner ? ex + ez : ex;
+}
 // Read-only tables indexed by a wave-uniform column (the x-profiles of the
 // sponge): in the constant address space the compiler reads them with scalar
 // loads, off the vector memory path that streams the state.

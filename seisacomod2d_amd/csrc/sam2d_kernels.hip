@@ -129,7 +129,9 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
         const SAM2D_GLOBAL float *__restrict__ pzi = gp(slot_pz(d, in_slot));
         SAM2D_GLOBAL float *__restrict__ pzo = gp(slot_pz(d, out_slot));
         SAM2D_GLOBAL char *__restrict__ ro =
-            RING ? (SAM2D_GLOBAL char *)gp(d.ring + (size_t)ring_slot * 3 * d.ncell) : nullptr;
+            RING ? (SAM2D_GLOBAL char *)gp(ring_pair(d.ring, d.ncell, ring_slot)) : nullptr;
+        SAM2D_GLOBAL float *__restrict__ rs =
+            RING ? gp(ring_side(d.ring, d.ncell, ring_slot)) : nullptr;
         const SAM2D_GLOBAL float *__restrict__ KX = gp(d.KX);
         const SAM2D_GLOBAL float *__restrict__ KZ = gp(d.KZ);
         const SAM2D_GLOBAL float *__restrict__ BX = gp(d.BX);
@@ -228,8 +230,15 @@ void k_step(const ShotDev *__restrict__ shots, int nshots, int vtotal, int per_x
                 pw[(u + 3) & 3] = pn;
                 pxnw[(u + 3) & 3] = inner ? pn : pxn;  // stored with vx, vz at j
                 pznw[(u + 3) & 3] = inner ? 0.0f : pzn;
-                if (RING && rown && c >= xa && c < xb)
-                    st3(ro, ((unsigned)c * (unsigned)nnz + (unsigned)r) * 12u, pn, vx_0, vz_0);
+                if (RING && rown && c >= xa && c < xb) {
+                    // the step's pressure and the dJ/dkappa factors of the
+                    // state before it (ring_pair, ring_side)
+                    float ez;
+                    const float e = ring_factor(dvx, d.dtdx * cur.skx, dvz, d.dtdz * skz, inner, ez);
+                    const size_t cell = (size_t)c * (size_t)nnz + (size_t)r;
+                    st2(ro + cell * 8, 0u, pn, e);
+                    if (!inner) rs[cell] = ez;
+                }
 
                 // ---- v-phase at column j = c - 2 (cpu_pml_v, :353-368)
                 const int j = c - 2;

@@ -103,11 +103,13 @@ __device__ __forceinline__ void stepn_tile(const ShotDev &d, int lane, const Ste
     gcchar *__restrict__ Bf = (gcchar *)d.B;
     const SAM2D_CONST float *__restrict__ prx = cp4(d.prof_x);
     const SAM2D_GLOBAL float *__restrict__ prz = gp(d.prof_z);
-    // Ring slots of the T steps (RING only).
+    // Ring slots of the T steps (RING only): the pairs, the frame's side
+    // values 8 ncell bytes further (ring_pair, ring_side).
     gchar *__restrict__ ring[T];
     #pragma unroll
     for (int t = 0; t < T; t++)
-        ring[t] = RING ? (gchar *)(d.ring + (size_t)(ring_slot + t) * 3 * d.ncell) : nullptr;
+        ring[t] = RING ? (gchar *)ring_pair(d.ring, d.ncell, ring_slot + t) : nullptr;
+    const unsigned rside = (unsigned)d.ncell * 8u;
 
     auto ld = [](gcchar *__restrict__ base, unsigned off) -> float {
         return *(const SAM2D_GLOBAL float *__restrict__)(base + off);
@@ -285,9 +287,16 @@ __device__ __forceinline__ void stepn_tile(const ShotDev &d, int lane, const Ste
                         if (band_lane) bandp[t][(size_t)r * nnx + cp] = P[t][i3];
                     }
                 }
-                if (RING && rown && cp >= xa && cp < xb)
-                    st3(ring[t], ((unsigned)cp * (unsigned)nnz + rc) * 12u, P[t][i3],
-                        vxw[i2], vz_0);
+                if (RING && rown && cp >= xa && cp < xb) {
+                    // the step's pressure and the dJ/dkappa factors of the
+                    // state before it (ring_pair, ring_side)
+                    const float skx = FAST ? 1.0f : prx[nnx + clampc(cp)];
+                    float ez;
+                    const float e = ring_factor(dvx, dtdx * skx, dvz, dtdz * skz, inner, ez);
+                    const unsigned cell = (unsigned)cp * (unsigned)nnz + rc;
+                    st2(ring[t], cell * 8u, P[t][i3], e);
+                    if (MODE != 2 && !inner) st1(ring[t], rside + cell * 4u, ez);
+                }
             }
             // ---- v-phase of step t at column jv
             const int jv = cp - 2;
@@ -402,10 +411,10 @@ __device__ __forceinline__ bool fast_tile(const ShotDev &d, int xa, int xb)
 
 // The kernel: T steps of every tile of every staged shot.
 // RING: the gradient's recomputation of a checkpoint segment.  For each step
-// t the p-phase also stores, for every owned cell of its column, the triplet
-// (p after step+t, vx and vz before it) into ring slot ring_slot + t: what
-// one backward step reads of the forward, 12 bytes per cell and step, in one
-// dwordx3 store per lane.  No receivers are recorded.
+// t the p-phase also stores, for every owned cell of its column, what one
+// backward step reads of the forward into ring slot ring_slot + t: the pair
+// (p after step+t, E) in one dwordx2 store per lane, and a frame lane's Ez
+// (ring_pair, ring_side).  No receivers are recorded.
 // Three-step kernels are held to three waves per SIMD (at most 168 VGPRs),
 // the occupancy their measured best shapes need.
 template <int T, int WAVES, int PF, bool KZA, bool RING>
