@@ -93,7 +93,7 @@ static int rec_sample(const TimeAxis &ta, int n)
 
 // Host chain rule of one shot's padded-grid gradients into the model grid.
 // Device bytes one shot of `plan` needs for a gradient with checkpoint
-// interval k: two state slots, k ring triplets, the checkpoints, the adjoint
+// interval k: two state slots, k ring slots, the checkpoints, the adjoint
 // state with its accumulators and the adjoint's coefficient cells.
 static size_t shot_bytes(const ShotPlan &p, int fd_nt, int k)
 {
@@ -391,7 +391,7 @@ static int gradient_batch(sam2d_ctx *ctx, std::vector<ShotPlan> &plans, const fl
     for (int m = nseg - 1; m >= 0 && backward; m--) {
         const int n0 = m * k, n1 = std::min(ta.fd_nt, n0 + k);
         SAM2D_TRY(save_or_restore(m, 0, false));
-        // Recompute the segment, step n's triplets into ring slot n - n0:
+        // Recompute the segment, step n's ring entries into ring slot n - n0:
         // three steps per launch, single steps for the rest.
         int n = n0, st = 0;
         if (ring_supported(ring_variant))

@@ -37,9 +37,9 @@ __device__ __forceinline__ void dplus_coef(int i, int n, float &c0, float &c1)
 typedef float fquad __attribute__((ext_vector_type(4)));
 template <class T>
 __device__ __forceinline__ SAM2D_GLOBAL T *gp(T *p) { return (SAM2D_GLOBAL T *)p; }
-// Three floats of a cell (the state's (P, vx, vz), the forward ring's
-// triplets, the adjoint's accumulators and coefficients), 4-byte aligned: one
-// dwordx3 access per lane.
+// Three floats of a cell (the state's (P, vx, vz), the adjoint's state,
+// accumulators and coefficients), 4-byte aligned: one dwordx3 access per
+// lane.
 typedef float ftri __attribute__((ext_vector_type(3), aligned(4)));
 __device__ __forceinline__ ftri ld3(const SAM2D_GLOBAL char *__restrict__ base, unsigned off)
 {

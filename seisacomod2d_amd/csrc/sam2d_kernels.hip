@@ -76,8 +76,9 @@ __device__ __forceinline__ void st3n(SAM2D_GLOBAL char *base, unsigned off, floa
 //
 // WAVES: z-strips per workgroup.  PF: columns of operands in flight (2 or 4).
 // NTS: non-temporal hint on the state stores.  RING (the gradient's
-// recomputation): the triplet (p after the step, vx and vz before it) of
-// every owned cell also goes to ring slot ring_slot.
+// recomputation): the ring entries of every owned cell (p after the step and
+// the dJ/dkappa factors of the state before it, ring_pair / ring_side) also go
+// to ring slot ring_slot.
 //
 // The march along x is unrolled by four columns so that the sliding windows
 // (vx at c-2..c+1, new p at c-3..c) rotate by compile-time index and the
