@@ -91,7 +91,9 @@ __device__ __forceinline__ float shfl_dn(float v, int d) { return __shfl_down(v,
 // Whole-wavefront lane shifts by DPP (wave_shr:1 = 0x138, wave_shl:1 = 0x130):
 // lane i receives lane i-1 (up) or lane i+1 (down); the lane without a source
 // receives 0 (bound_ctrl).  One VALU operation, no LDS; all 64 lanes must be
-// active.
+// active: a disabled source lane also reads as 0, so a shift is never written
+// inside a divergent branch, and that includes one arm of ?: (a convergent
+// call keeps clang's branch for the arms).
 __device__ __forceinline__ float lane_up1(float v)
 {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));

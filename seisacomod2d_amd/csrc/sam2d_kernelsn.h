@@ -316,13 +316,18 @@ __device__ __forceinline__ void stepn_tile(const ShotDev &d, int lane, const Ste
                     // in FP64, rounded without bias)
                     const float b_1 = keep(cur.b);            // b(jv + 1)
                     const bool last_row = r >= nnz - 1;
-                    const float bu1 = last_row ? 0.0f : 0.5f * (lane_dn1(b_0) + b_0);
+                    // The lane shifts run before the row test: inside an arm
+                    // of ?: they would run with the last row's lanes disabled,
+                    // which read as 0 to the row above.
+                    const float b_d1 = lane_dn1(b_0);         // b(jv, r + 1)
+                    const float bu1 = last_row ? 0.0f : 0.5f * (b_d1 + b_0);
                     float bu2;
                     if (FAST || jv < nnx - 1) {
                         bu2 = 0.5f * (b_1 + b_0);
                     } else {        // last column: bu1 of column nnx - 2
                         const float b_m1 = ld(Bf, coff(jv - 1));
-                        bu2 = last_row ? 0.0f : 0.5f * (lane_dn1(b_m1) + b_m1);
+                        const float b_m1d1 = lane_dn1(b_m1);
+                        bu2 = last_row ? 0.0f : 0.5f * (b_m1d1 + b_m1);
                     }
                     bx = round_unbiased_prod_pos(dtdx, bu2, cell_draw(RND_BX, jv, r));
                     bz = round_unbiased_prod_pos(dtdz, bu1, cell_draw(RND_BZ, jv, r));
