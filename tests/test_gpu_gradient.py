@@ -43,15 +43,20 @@ def _oracle_grad(oracle, grid, vp, rho, shots, adj):
     return P, oracle.gradient(P, vp, rho, [[s.sx, s.sz] for s in shots], [s.rec_xz for s in shots], adj)
 
 
-def test_gradient_vs_fp64_oracle(oracle):
-    grid, vp, rho, shots = _setup()
+@pytest.mark.parametrize("npml,T", [(8, 0.2), (2, 0.35)])
+def test_gradient_vs_fp64_oracle(oracle, npml, T):
+    # npml = 2: the wave reaches the last rows and columns, whose edge rules
+    # a thick sponge damps away
+    grid, vp, rho, shots = _setup(npml=npml, T=T)
     rng = np.random.default_rng(1)
     with sam.Modeller(grid, vp, rho) as m:
         adj = [rng.standard_normal((s.nrec, m.real_nt)).astype(np.float32) for s in shots]
         gvp, grho = m.gradient(shots, adj, ckpt_k=13)
     P, (ovp, orho) = _oracle_grad(oracle, grid, vp, rho, shots, adj)
     print(f"[gpu] gradient rel L2: vp {_rel(gvp, ovp):.2e} rho {_rel(grho, orho):.2e}", flush=True)
-    assert _rel(gvp, ovp) <= 1e-4 and _rel(grho, orho) <= 1e-4
+    # about 1e-6 measured; BZ of row nnz-2 halved in the recomputation gave 4e-5
+    # with npml = 2
+    assert _rel(gvp, ovp) <= 1e-5 and _rel(grho, orho) <= 1e-5
 
 
 def test_gradient_directional_derivatives_fp64(oracle):
