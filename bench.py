@@ -303,10 +303,19 @@ def bench_c1_batched(args, sam, nb=8):
     marches (less x-halo) and each launch drains once for nb shots.  Reported
     beside the per-shot loop, which stays the headline value."""
     vp, rho = model()
-    grid = sam.Grid(NX, NZ, H, H, T_REC, DT, DT, FM, npml=NPML)
+    t_rec = T_REC if args.profile_steps <= 0 else args.profile_steps * DT
+    grid = sam.Grid(NX, NZ, H, H, t_rec, DT, DT, FM, npml=NPML)
     sh = shots(sam)[:nb]
+    variant = str(args.variant)
+    variant = int(variant) if variant.lstrip("-").isdigit() else sam.variants().index(variant)
     with sam.Modeller(grid, vp, rho) as m:
-        best = m.autotune(nshots=nb, steps=6)
+        if variant >= 0:                       # a fixed configuration (profiling runs)
+            best = variant
+            m.set_option("variant", variant)
+            m.set_option("tile_order", max(args.tile_order, 0))
+            m.set_option("chunk", max(args.chunk, 0))
+        else:
+            best = m.autotune(nshots=nb, steps=6)
         m.forward_batched(sh)                  # warm: the time loop's graph captured
         t0 = time.perf_counter()
         m.forward_batched(sh)
@@ -318,7 +327,7 @@ def bench_c1_batched(args, sam, nb=8):
     us = st["kernel_ms"] * 1e3 / st["launches"]
     shot_launch_us = us / nb
     frac = BYTES_PER_CELL * cells / (shot_launch_us * 1e-6) / 1e9 / HBM_PEAK_GBS
-    return {"workload": f"config1_5000x5000_nt4000_npml50_batched_{nb}_shots",
+    return {"workload": f"config1_5000x5000_nt{fd_nt}_npml50_batched_{nb}_shots",
             "shots": nb, "gcells_wall": round(cells * fd_nt * nb / wall / 1e9, 2),
             "gcells_kernel": round(cells * fd_nt * nb / (st["kernel_ms"] * 1e-3) / 1e9, 2),
             "launch_us": round(us, 2), "us_per_shot_launch": round(shot_launch_us, 2),
@@ -512,12 +521,14 @@ def main():
     ap.add_argument("--hbm-budget-gb", type=int, default=0,
                     help="gradient workload: device memory a gradient batch may use "
                          "(0: the library's default)")
-    ap.add_argument("--workload", default="plain", choices=["plain", "batched", "gradient"])
+    ap.add_argument("--workload", default="plain",
+                    choices=["plain", "batched", "gradient", "config1-batched"])
     ap.add_argument("--quick", action="store_true", help="batched/gradient: fewer shots")
     ap.add_argument("--max-batch", type=int, default=0)
     ap.add_argument("--c4-only", action="store_true", help="gradient workload: the salt model only")
     ap.add_argument("--ckpt-k", default="", help="gradient workload: checkpoint intervals, e.g. 100")
-    ap.add_argument("--shots", type=int, default=0, help="gradient workload: salt-model shots")
+    ap.add_argument("--shots", type=int, default=0,
+                    help="gradient workload: salt-model shots; config1-batched: shots per launch")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="profiling runs only: shorten each shot (each gradient record) to "
                          "this many time steps")
@@ -539,6 +550,9 @@ def main():
         return
     if args.workload == "gradient":
         print(json.dumps(bench_gradient(args, sam)), flush=True)
+        return
+    if args.workload == "config1-batched":
+        print(json.dumps(bench_c1_batched(args, sam, args.shots or 8)), flush=True)
         return
     out = bench_plain(args, sam)
     if not args.no_extra and args.profile_steps <= 0:
