@@ -123,3 +123,28 @@ def test_config4_full_gradient_batch_split_by_the_memory_budget_bitwise():
     for g in (halves, split):
         assert np.array_equal(g[0].view(np.int32), one[0].view(np.int32))
         assert np.array_equal(g[1].view(np.int32), one[1].view(np.int32))
+
+
+def test_config1_batched_shots_equal_single_step_bitwise():
+    # The C1 shots through the batched forward as the benchmark runs them
+    # (eight per launch, the three-step kernel with 193-column marches, tile
+    # order 9), 300 steps, against the single-step kernel one shot at a time,
+    # every trace bit for bit.
+    from seisacomod2d_amd import synthetic
+    vp, rho = synthetic.two_layer(5000, 5000)
+    shots = [sam.Shot(sx, sz, rec) for sx, sz, rec in
+             synthetic.surface_spread_shots(5000, 10.0, 64)][:8]
+    grid = sam.Grid(5000, 5000, 10.0, 10.0, 0.3, 0.001, 0.001, 15.0, npml=50)
+    print("[gpu] config1 batched, 8 shots x 300 steps", flush=True)
+    with sam.Modeller(grid, vp, rho) as m:
+        m.set_option("variant", sam.variants().index("threestep_waves4_chunk256_pf4"))
+        m.set_option("chunk", 193)
+        m.set_option("tile_order", 9)
+        batched = m.forward_batched(shots)
+        m.set_option("variant", sam.variants().index("waves4_chunk32_pf2"))
+        m.set_option("chunk", 0)
+        m.set_option("tile_order", 0)
+        loop = [m.forward([s])[0] for s in shots]
+    for s, (a, b) in enumerate(zip(batched, loop)):
+        assert np.array_equal(a.view(np.int32), b.view(np.int32)), s
+        assert np.all(np.isfinite(a)) and np.abs(a).max() > 0
